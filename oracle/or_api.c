@@ -1,0 +1,111 @@
+/*
+ * Flat entry points of the oracle for ctypes (same shapes as ref_driver.c),
+ * so tests drive the oracle and the compiled reference identically.
+ * TEST INFRASTRUCTURE ONLY.
+ */
+#include "dsd_oracle.h"
+#include <stdlib.h>
+#include <string.h>
+
+int ora_ntf_count(void) { return or_ntf_count(); }
+
+int ora_ntf_get(int i, double *a, double *g, int *order, unsigned *freq,
+                char *name, int name_len)
+{
+    const or_ntf_t *f = or_ntf_get(i);
+    if (!f) return -1;
+    memcpy(a, f->a, sizeof(f->a));
+    memcpy(g, f->g, sizeof(f->g));
+    *order = f->order;
+    *freq = f->freq;
+    strncpy(name, f->name, (size_t)name_len - 1);
+    name[name_len - 1] = 0;
+    return 0;
+}
+
+int ora_ntf_lookup(int id, unsigned rate) { return or_ntf_lookup(id, rate); }
+int ora_ntf_auto(unsigned rate) { return or_ntf_auto(rate); }
+
+void *ora_ctx_new(int filter_index, int depth, int cands, int lat, double state_limit)
+{
+    or_ctx *c = (or_ctx *)calloc(1, sizeof(or_ctx));
+    if (!c) return NULL;
+    if (or_sdm_init(c, or_ntf_get(filter_index), depth, cands, lat) != 0) {
+        free(c);
+        return NULL;
+    }
+    c->state_limit = state_limit;
+    return c;
+}
+
+void ora_ctx_free(void *c) { free(c); }
+void ora_ctx_reset(void *c) { or_sdm_reset((or_ctx *)c); }
+
+size_t ora_process(void *c, const double *in, float *out, size_t n)
+{
+    return or_sdm_process((or_ctx *)c, in, out, n);
+}
+
+size_t ora_drain(void *c, float *out, size_t max_out)
+{
+    return or_sdm_drain((or_ctx *)c, out, max_out);
+}
+
+void ora_copy_state(void *dst, const void *src)
+{
+    or_sdm_copy_state((or_ctx *)dst, (const or_ctx *)src);
+}
+
+void ora_best_state(const void *c, double *out) { or_sdm_best_state((const or_ctx *)c, out); }
+
+void ora_seed(void *cv, const double *st)
+{
+    or_ctx *c = (or_ctx *)cv;
+    int bank = c->idx & 1;
+    for (unsigned i = 0; i < c->num_cands; i++) {
+        or_node *s = c->act[bank][i];
+        if (s) {
+            for (int k = 0; k < c->f->order; k++) s->st[k] = st[k];
+            s->cost = 0.0;
+        }
+    }
+}
+
+void ora_counters(const void *cv, unsigned long long *out)
+{
+    const or_ctx *c = (const or_ctx *)cv;
+    out[0] = c->num_cands;
+    out[1] = c->pos;
+    out[2] = c->pending;
+    out[3] = c->idx;
+    out[4] = c->conv_fail;
+    out[5] = c->cands_collapse;
+    out[6] = c->next_filter_drops;
+    out[7] = c->total_children;
+}
+
+void ora_estimate(int filter_index, const double *init, const double *in,
+                  size_t count, double state_limit, int levels, double *out)
+{
+    or_estimate_state(or_ntf_get(filter_index), init, in, count, state_limit, levels, out);
+}
+
+size_t ora_segment(void *c, const double *in, float *out, size_t count, size_t discard)
+{
+    return or_sdm_segment((or_ctx *)c, in, out, count, discard);
+}
+
+size_t ora_ctx_size(void) { return sizeof(or_ctx); }
+
+/* (bank * 256 + slot) of the node the first hash bucket points at, -1 if
+ * empty; matches ref_hash0_offset decoded with the reference's layout. */
+long ora_hash0_slot(const void *cv)
+{
+    const or_ctx *c = (const or_ctx *)cv;
+    const or_node *t = c->head[0];
+    if (!t) return -1;
+    for (int b = 0; b < 2; b++)
+        if (t >= &c->node[b][0] && t < &c->node[b][OR_NODES])
+            return b * 256 + (long)(t - &c->node[b][0]);
+    return -2;   /* points into another context */
+}

@@ -1,0 +1,153 @@
+"""ctypes access to the CPU oracle (oracle/liboracle.so) and, where it was
+built, the compiled reference trellis core (oracle/_ref/libdsdref.so).
+
+Test infrastructure only: the product package never imports this module.
+Both libraries expose the same flat API (prefix `ora_` / `ref_`), so a test
+can run the oracle and the reference on identical inputs.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libdsdref.so")
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+class TrellisLib:
+    """Wrapper over one of the two libraries (prefix 'ora' or 'ref')."""
+
+    def __init__(self, path, prefix):
+        self.lib = C.CDLL(path)
+        self.p = prefix
+        L = self.lib
+        f = lambda n: getattr(L, "%s_%s" % (prefix, n))
+        f("ntf_get").argtypes = [C.c_int, _dp, _dp, C.POINTER(C.c_int), C.POINTER(C.c_uint), C.c_char_p, C.c_int]
+        f("ntf_lookup").argtypes = [C.c_int, C.c_uint]
+        f("ntf_auto").argtypes = [C.c_uint]
+        f("ctx_new").argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
+        f("ctx_new").restype = C.c_void_p
+        f("ctx_free").argtypes = [C.c_void_p]
+        f("ctx_reset").argtypes = [C.c_void_p]
+        f("process").argtypes = [C.c_void_p, _dp, _fp, C.c_size_t]
+        f("process").restype = C.c_size_t
+        f("drain").argtypes = [C.c_void_p, _fp, C.c_size_t]
+        f("drain").restype = C.c_size_t
+        f("copy_state").argtypes = [C.c_void_p, C.c_void_p]
+        f("best_state").argtypes = [C.c_void_p, _dp]
+        f("seed").argtypes = [C.c_void_p, _dp]
+        f("counters").argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong)]
+        f("estimate").argtypes = [C.c_int, _dp, _dp, C.c_size_t, C.c_double, C.c_int, _dp]
+        self._f = f
+
+    def ntf_count(self):
+        return self._f("ntf_count")()
+
+    def ntf_get(self, i):
+        a = np.zeros(10)
+        g = np.zeros(10)
+        order = C.c_int()
+        freq = C.c_uint()
+        name = C.create_string_buffer(32)
+        rc = self._f("ntf_get")(i, _ptr(a, _dp), _ptr(g, _dp), C.byref(order), C.byref(freq), name, 32)
+        if rc:
+            return None
+        return dict(a=a, g=g, order=order.value, freq=freq.value, name=name.value.decode())
+
+    def ntf_lookup(self, ntf_id, rate):
+        return self._f("ntf_lookup")(ntf_id, rate)
+
+    def ntf_auto(self, rate):
+        return self._f("ntf_auto")(rate)
+
+    def ctx(self, filter_index, depth, cands, lat, state_limit=0.0):
+        h = self._f("ctx_new")(filter_index, depth, cands, lat, state_limit)
+        if not h:
+            raise ValueError("context init rejected")
+        return Ctx(self, h)
+
+    def estimate(self, filter_index, init, x, state_limit=0.0, levels=16):
+        init = np.ascontiguousarray(init, dtype=np.float64)
+        if init.size < 10:
+            init = np.concatenate([init, np.zeros(10 - init.size)])
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(10)
+        self._f("estimate")(filter_index, _ptr(init, _dp), _ptr(x, _dp), x.size, state_limit, levels, _ptr(out, _dp))
+        return out
+
+
+class Ctx:
+    def __init__(self, lib, h):
+        self.lib = lib
+        self.h = h
+
+    def __del__(self):
+        try:
+            self.lib._f("ctx_free")(self.h)
+        except Exception:
+            pass
+
+    def process(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(max(x.size, 1), dtype=np.float32)
+        n = self.lib._f("process")(self.h, _ptr(x, _dp), _ptr(out, _fp), x.size)
+        return out[:n]
+
+    def drain(self, max_out=1 << 20):
+        out = np.zeros(max_out, dtype=np.float32)
+        n = self.lib._f("drain")(self.h, _ptr(out, _fp), max_out)
+        return out[:n]
+
+    def reset(self):
+        self.lib._f("ctx_reset")(self.h)
+
+    def copy_from(self, other):
+        self.lib._f("copy_state")(self.h, other.h)
+
+    def best_state(self):
+        out = np.zeros(10)
+        self.lib._f("best_state")(self.h, _ptr(out, _dp))
+        return out
+
+    def seed(self, st):
+        st = np.ascontiguousarray(st, dtype=np.float64)
+        if st.size < 10:
+            st = np.concatenate([st, np.zeros(10 - st.size)])
+        self.lib._f("seed")(self.h, _ptr(st, _dp))
+
+    def counters(self):
+        out = (C.c_ulonglong * 8)()
+        self.lib._f("counters")(self.h, out)
+        keys = ["num_cands", "pos", "pending", "idx", "conv_fail", "cands_collapse",
+                "next_filter_drops", "total_children"]
+        return dict(zip(keys, list(out)))
+
+
+_cache = {}
+
+
+def oracle():
+    if "ora" not in _cache:
+        _cache["ora"] = TrellisLib(ORACLE_SO, "ora")
+    return _cache["ora"]
+
+
+def reference():
+    """The compiled reference, or None where its sources were not present."""
+    if "ref" not in _cache:
+        _cache["ref"] = TrellisLib(REF_SO, "ref") if os.path.exists(REF_SO) else None
+    return _cache["ref"]
+
+
+def pack_bits(pm1):
+    """+-1.0 float stream -> packed little-endian bits (1 for +1)."""
+    b = (np.asarray(pm1) > 0).astype(np.uint8)
+    return np.packbits(b, bitorder="little")
