@@ -1,0 +1,274 @@
+/*
+ * Host side of the trellis backend: device context pool, the reference's
+ * sdm_* operations expressed on it, and batched job launch.
+ *
+ * Reference operations and where they live here:
+ *   sdm_context_init / reset      (src/trellis.c:977-1011, 1189-1224)  -> dsdhip_trellis_reset
+ *   sdm_process_block             (src/trellis.c:1013-1040)            -> dsdhip_trellis_run
+ *   sdm_context_copy_state        (src/trellis.c:1064-1092)            -> dsdhip_trellis_copy
+ *   DAS candidate seeding         (src/dsp_plugin.c:2329-2338)         -> dsdhip_trellis_seed
+ */
+#include <hip/hip_runtime.h>
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+#include "dsdhip_internal.h"
+
+extern "C" int tr_launch_jobs(const tr_params *prm, tr_job *d_jobs, int njobs,
+                              tr_ctx *d_ctxs, hipStream_t stream);
+
+struct dsdhip_trellis {
+    tr_params prm;
+    int nctx;
+    tr_ctx *d_ctx;             /* device context pool */
+    std::vector<int> snap_of;  /* context each one was copied from (-1 none) */
+    tr_job *d_jobs;
+    int jobs_cap;
+    hipStream_t stream;
+};
+
+static void ctx_fresh(tr_ctx *c)
+{
+    memset(c, 0, sizeof(*c));
+    c->num_cands = 1;
+    memset(c->link, 0xff, sizeof(c->link));
+}
+
+extern "C" {
+
+dsdhip_trellis_t *dsdhip_trellis_create_coeffs(const double *a, const double *g, int order,
+                                               int depth, int cands, int lat,
+                                               double state_limit, int nctx)
+{
+    if (!a || !g || nctx < 1) return NULL;
+    if (depth < 1 || depth > 32 || cands < 1 || cands > TR_MAXK || lat < 1 || lat > TR_MAXL)
+        return NULL;
+    if (order < 4 || order > TR_MAXP) return NULL;
+    if (dsdhip_ensure_device() != 0) return NULL;
+    dsdhip_trellis_t *t = new dsdhip_trellis();
+    memset(&t->prm, 0, sizeof(t->prm));
+    for (int i = 0; i < order; i++) {
+        t->prm.a[i] = a[i];
+        t->prm.g[i] = g[i];
+    }
+    t->prm.P = order;
+    t->prm.K = cands;
+    t->prm.L = lat;
+    t->prm.B = depth;
+    t->prm.mask = (uint32_t)(((uint64_t)1 << depth) - 1);
+    t->prm.state_limit = state_limit;
+    t->nctx = nctx;
+    t->snap_of.assign(nctx, -1);
+    t->d_jobs = NULL;
+    t->jobs_cap = 0;
+    t->stream = 0;
+    if (hipMalloc(&t->d_ctx, sizeof(tr_ctx) * (size_t)nctx) != hipSuccess) {
+        delete t;
+        return NULL;
+    }
+    for (int i = 0; i < nctx; i++) dsdhip_trellis_reset(t, i);
+    return t;
+}
+
+dsdhip_trellis_t *dsdhip_trellis_create(int ntf_index, int depth, int cands, int lat,
+                                        double state_limit, int nctx)
+{
+    const dsdhip_ntf_t *f = dsdhip_ntf_get(ntf_index);
+    if (!f) return NULL;
+    return dsdhip_trellis_create_coeffs(f->a, f->g, f->order, depth, cands, lat, state_limit, nctx);
+}
+
+void dsdhip_trellis_destroy(dsdhip_trellis_t *t)
+{
+    if (!t) return;
+    hipFree(t->d_ctx);
+    if (t->d_jobs) hipFree(t->d_jobs);
+    delete t;
+}
+
+int dsdhip_trellis_set_stream(dsdhip_trellis_t *t, void *stream)
+{
+    if (!t) return -1;
+    t->stream = (hipStream_t)stream;
+    return 0;
+}
+
+int dsdhip_trellis_force_literal(dsdhip_trellis_t *t, int on)
+{
+    if (!t) return -1;
+    t->prm.force_slow = on ? 1 : 0;
+    return 0;
+}
+
+int dsdhip_trellis_num_contexts(const dsdhip_trellis_t *t) { return t ? t->nctx : 0; }
+
+void *dsdhip_trellis_context_ptr(dsdhip_trellis_t *t, int i)
+{
+    if (!t || i < 0 || i >= t->nctx) return NULL;
+    return &t->d_ctx[i];
+}
+
+int dsdhip_trellis_reset(dsdhip_trellis_t *t, int i)
+{
+    if (!t || i < 0 || i >= t->nctx) return -1;
+    tr_ctx c;
+    ctx_fresh(&c);
+    t->snap_of[i] = -1;
+    return hipMemcpy(&t->d_ctx[i], &c, sizeof(c), hipMemcpyHostToDevice) == hipSuccess ? 0 : -1;
+}
+
+int dsdhip_trellis_get(dsdhip_trellis_t *t, int i, tr_ctx *out)
+{
+    if (!t || i < 0 || i >= t->nctx || !out) return -1;
+    if (hipStreamSynchronize(t->stream) != hipSuccess) return -1;
+    return hipMemcpy(out, &t->d_ctx[i], sizeof(*out), hipMemcpyDeviceToHost) == hipSuccess ? 0 : -1;
+}
+
+int dsdhip_trellis_put(dsdhip_trellis_t *t, int i, const tr_ctx *in)
+{
+    if (!t || i < 0 || i >= t->nctx || !in) return -1;
+    return hipMemcpy(&t->d_ctx[i], in, sizeof(*in), hipMemcpyHostToDevice) == hipSuccess ? 0 : -1;
+}
+
+/* sdm_context_copy_state: dst becomes a copy of src; its hash links keep
+ * pointing into src, so later lookups through them read src's slots. */
+int dsdhip_trellis_copy(dsdhip_trellis_t *t, int dst, int src)
+{
+    if (!t || dst < 0 || src < 0 || dst >= t->nctx || src >= t->nctx || dst == src) return -1;
+    tr_ctx c;
+    if (dsdhip_trellis_get(t, src, &c) != 0) return -1;
+    int bad = 0;
+    for (int b = 0; b < 2; b++)
+        for (int i = 0; i < TR_SLOTS; i++) {
+            int8_t l = c.link[b][i];
+            if (l >= TR_LINK_FOREIGN) bad = 1;
+            else if (l >= 0) c.link[b][i] = (int8_t)(l + TR_LINK_FOREIGN);
+        }
+    c.has_foreign = 1;
+    if (bad) return -1;   /* links two contexts deep are not represented */
+    t->snap_of[dst] = src;
+    return dsdhip_trellis_put(t, dst, &c);
+}
+
+int dsdhip_trellis_seed(dsdhip_trellis_t *t, int i, const double *st)
+{
+    tr_ctx c;
+    if (dsdhip_trellis_get(t, i, &c) != 0) return -1;
+    for (int k = 0; k < c.num_cands; k++) {
+        for (int q = 0; q < t->prm.P; q++) c.st[k][q] = st[q];
+        c.cost[k] = 0.0;
+    }
+    return dsdhip_trellis_put(t, i, &c);
+}
+
+int dsdhip_trellis_best_state(dsdhip_trellis_t *t, int i, double *out)
+{
+    tr_ctx c;
+    if (dsdhip_trellis_get(t, i, &c) != 0) return -1;
+    for (int q = 0; q < 10; q++) out[q] = q < t->prm.P ? c.st[0][q] : 0.0;
+    return 0;
+}
+
+/*
+ * Run a batch of jobs in one launch.  d_in[j] / d_out[j] are device
+ * pointers; job j runs n_in[j] samples on context ctx[j] (optionally
+ * starting from a copy of copy_src[j] >= 0 seeded with seed[j*10..]),
+ * keeps outputs produced from input index keep_from[j] on, and stores the
+ * final state back into ctx[j].  kept[j] receives the kept output count.
+ * Returns 0, or -1 on launch failure / error flag (err_out gets the flag).
+ */
+int dsdhip_trellis_run_jobs(dsdhip_trellis_t *t, int njobs, const int *ctx, const int *copy_src,
+                            const double *const *d_in, const int64_t *n_in,
+                            const int64_t *keep_from, uint64_t *const *d_out,
+                            const double *seed, int64_t *kept, int64_t *slow_steps,
+                            int *err_out)
+{
+    if (!t || njobs < 0) return -1;
+    if (njobs == 0) return 0;
+    std::vector<tr_job> jobs((size_t)njobs);
+    for (int j = 0; j < njobs; j++) {
+        tr_job &jb = jobs[(size_t)j];
+        memset(&jb, 0, sizeof(jb));
+        if (ctx[j] < 0 || ctx[j] >= t->nctx) return -1;
+        int src = (copy_src && copy_src[j] >= 0) ? copy_src[j] : -1;
+        jb.in = d_in[j];
+        jb.out = d_out[j];
+        jb.n_in = n_in[j];
+        jb.keep_from = keep_from ? keep_from[j] : 0;
+        jb.ctx_out = ctx[j];
+        if (src >= 0) {
+            if (src >= t->nctx) return -1;
+            jb.ctx_in = src;
+            jb.copy = 1;
+            jb.snap = src;
+        } else {
+            jb.ctx_in = ctx[j];
+            jb.copy = 0;
+            jb.snap = t->snap_of[ctx[j]];
+        }
+        if (seed && !(seed[j * 10] != seed[j * 10])) {   /* NaN in slot 0 = no seed */
+            jb.seed = 1;
+            for (int q = 0; q < TR_MAXP; q++) jb.seed_st[q] = seed[j * 10 + q];
+        }
+    }
+    if (t->jobs_cap < njobs) {
+        if (t->d_jobs) hipFree(t->d_jobs);
+        t->d_jobs = NULL;
+        t->jobs_cap = 0;
+        if (hipMalloc(&t->d_jobs, sizeof(tr_job) * (size_t)njobs) != hipSuccess) return -1;
+        t->jobs_cap = njobs;
+    }
+    if (hipMemcpyAsync(t->d_jobs, jobs.data(), sizeof(tr_job) * (size_t)njobs,
+                       hipMemcpyHostToDevice, t->stream) != hipSuccess)
+        return -1;
+    if (tr_launch_jobs(&t->prm, t->d_jobs, njobs, t->d_ctx, t->stream) != 0) return -1;
+    if (hipMemcpyAsync(jobs.data(), t->d_jobs, sizeof(tr_job) * (size_t)njobs,
+                       hipMemcpyDeviceToHost, t->stream) != hipSuccess)
+        return -1;
+    if (hipStreamSynchronize(t->stream) != hipSuccess) return -1;
+    int err = 0;
+    for (int j = 0; j < njobs; j++) {
+        if (kept) kept[j] = jobs[(size_t)j].kept;
+        if (slow_steps) slow_steps[j] = jobs[(size_t)j].slow_steps;
+        if (jobs[(size_t)j].err && !err) err = jobs[(size_t)j].err;
+        int src = (copy_src && copy_src[j] >= 0) ? copy_src[j] : -1;
+        if (src >= 0) t->snap_of[ctx[j]] = src;
+    }
+    if (err_out) *err_out = err;
+    return err ? -1 : 0;
+}
+
+/* sdm_process_block on one context with host buffers: float +-1.0 out. */
+int dsdhip_trellis_process(dsdhip_trellis_t *t, int i, const double *in, float *out,
+                           size_t count, size_t *out_count)
+{
+    if (!t || i < 0 || i >= t->nctx) return -1;
+    double *d_in = NULL;
+    uint64_t *d_out = NULL;
+    size_t nw = (count + 63) / 64 + 1;
+    if (hipMalloc(&d_in, sizeof(double) * (count ? count : 1)) != hipSuccess) return -1;
+    if (hipMalloc(&d_out, sizeof(uint64_t) * nw) != hipSuccess) { hipFree(d_in); return -1; }
+    int rc = -1;
+    if (hipMemcpy(d_in, in, sizeof(double) * count, hipMemcpyHostToDevice) == hipSuccess) {
+        const double *pin = d_in;
+        uint64_t *pout = d_out;
+        int64_t n = (int64_t)count, kf = 0, kept = 0;
+        int err = 0;
+        rc = dsdhip_trellis_run_jobs(t, 1, &i, NULL, &pin, &n, &kf, &pout, NULL, &kept, NULL, &err);
+        if (rc == 0) {
+            std::vector<uint64_t> bitsv(nw);
+            if (hipMemcpy(bitsv.data(), d_out, sizeof(uint64_t) * nw, hipMemcpyDeviceToHost) != hipSuccess)
+                rc = -1;
+            else {
+                for (int64_t k = 0; k < kept; k++)
+                    out[k] = ((bitsv[(size_t)(k >> 6)] >> (k & 63)) & 1) ? 1.0f : -1.0f;
+                if (out_count) *out_count = (size_t)kept;
+            }
+        }
+    }
+    hipFree(d_in);
+    hipFree(d_out);
+    return rc;
+}
+
+} /* extern "C" */

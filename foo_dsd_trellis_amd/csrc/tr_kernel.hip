@@ -1,0 +1,539 @@
+/*
+ * Trellis sigma-delta modulator for gfx950: one wave64 per trellis context.
+ *
+ * Lane j holds child j of the current step (parent = survivor j/2, branch
+ * bit j&1), so a full K=32 trellis fills the wave.  Per input sample:
+ *   1. expand   : each lane runs its parent's CRFB loop filter in the
+ *                 reference's scalar operation order (FP64, no contraction,
+ *                 src/trellis.c:37-191) and forms the child metric;
+ *   2. prune    : a wave-parallel formulation of the reference's sequential
+ *                 candidate sort (src/trellis.c:621-731): strict cost-order
+ *                 masks from an LDS broadcast sweep, majority/traceback
+ *                 filter, then a fixed-point resolution of the early skip
+ *                 and of the duplicate-path rules, and a final rank;
+ *                 steps that fall outside what the parallel form decides
+ *                 (equal metrics, a bucket-0 lookup after the full-list
+ *                 write past act[31] at K=32, paths wider than 8 bits) are
+ *                 decided by a literal sequential prune run by lane 0 over
+ *                 the same LDS state;
+ *   3. commit   : survivors write state/metric/path/history for the next
+ *                 step into the other LDS bank; the traceback bit of the
+ *                 best survivor is the output (src/trellis.c:938-972).
+ * History is carried by value per survivor (HW 64-bit words of L bits).
+ */
+#include <hip/hip_runtime.h>
+#include "tr_types.h"
+
+typedef unsigned long long u64;
+
+namespace {
+
+__device__ __forceinline__ double rl_d(double v, int l)
+{
+    u64 b = (u64)__double_as_longlong(v);
+    unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((u64)hi << 32) | lo));
+}
+
+__device__ __forceinline__ int rl_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ u64 ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+__device__ __forceinline__ int ff1(u64 m) { return (int)__builtin_ctzll(m); }
+__device__ __forceinline__ long long bits(double d) { return __double_as_longlong(d); }
+
+/* count of set bits of `m` below this lane */
+__device__ __forceinline__ int mbcnt(u64 m)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+__device__ __forceinline__ void wsync() { __syncthreads(); }
+
+/* Strict-order mask against the first NJ metrics of the LDS key array
+ * (broadcast reads, fully unrolled). */
+template <int NJ>
+__device__ __forceinline__ u64 lt_sweep(const double *key, double c)
+{
+    unsigned lo = 0, hi = 0;
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        const bool b = key[j] < c;
+        if (j < 32) lo |= b ? (1u << (j & 31)) : 0u;
+        else hi |= b ? (1u << (j & 31)) : 0u;
+    }
+    return ((u64)hi << 32) | lo;
+}
+
+/* LDS working set of one context. */
+template <int P, int HW>
+struct TrLds {
+    double st[2][TR_MAXK][P];
+    double cost[2][TR_MAXK];
+    u64 hist[2][TR_MAXK][HW];
+    uint32_t path[2][TR_MAXK];
+    uint8_t next[2][TR_MAXK];
+    double key[64];
+    u64 gtab[256];
+    u64 btab[128];
+    uint32_t npath[2][64];
+    double ncost[2][64];
+    int8_t link[2][64];
+    uint8_t rank[64];
+    uint8_t nnext[64];
+    int8_t head[128];
+    int8_t act[TR_MAXK + 1];
+    int32_t scal[4];
+};
+
+/* Literal sequential prune (src/trellis.c:582-731), run by one lane over the
+ * children held in LDS.  Writes rank[] (255 = pruned) and returns the number
+ * of survivors; updates hash links of newly registered children. */
+template <int P, int HW>
+__device__ int literal_prune(TrLds<P, HW> &s, int nb, int m, int K,
+                             const tr_ctx *snap, int *err)
+{
+    const uint32_t *np = s.npath[nb];
+    const double *nc = s.ncost[nb];
+    for (int i = 0; i < m; i++) s.head[np[i] & 127] = -1;
+    int mn = 0;
+    for (int i = 1; i < m; i++)
+        if (bits(nc[i]) < bits(nc[mn])) mn = i;
+    int v1 = 0;
+    for (int i = 0; i < m; i++) v1 += s.nnext[i] & 1;
+    int maj = (v1 > m - v1) ? 1 : 0;
+    if (s.nnext[mn] != maj) {
+        int bm = -1;
+        for (int i = 0; i < m; i++)
+            if (s.nnext[i] == maj && (bm < 0 || bits(nc[i]) < bits(nc[bm]))) bm = i;
+        if (bm >= 0 && nc[bm] < nc[mn] * 1.1) mn = bm;
+    }
+    const int M = s.nnext[mn];
+    int n = 0;
+    for (int i = 0; i < m; i++) {
+        if (s.nnext[i] != M) continue;
+        const long long ci = bits(nc[i]);
+        if (n == K && bits(nc[s.act[n - 1]]) <= ci) continue;
+        /* lookup along the hash chain */
+        const uint32_t pi = np[i];
+        const int b = (int)(pi & 127);
+        int t = -1;
+        long long tcost = 0;
+        int v = s.head[b];
+        int hops = 0;
+        while (v != -1) {
+            if (++hops > 2 * TR_SLOTS + 2) { *err = 2; break; }   /* links only point downwards */
+            uint32_t vp;
+            long long vc;
+            int vl;
+            if (v < TR_LINK_FOREIGN) {
+                vp = np[v];
+                vc = bits(nc[v]);
+                vl = s.link[nb][v];
+            } else {
+                if (!snap) { *err = 2; break; }
+                const int sv = v - TR_LINK_FOREIGN;
+                vp = snap->node_path[nb][sv];
+                vc = bits(snap->node_cost[nb][sv]);
+                vl = snap->link[nb][sv];
+                if (vl >= TR_LINK_FOREIGN) { *err = 2; break; }
+                if (vl >= 0) vl += TR_LINK_FOREIGN;
+            }
+            if (vp == pi) { t = v; tcost = vc; break; }
+            v = vl;
+        }
+        if (t < 0) {
+            s.link[nb][i] = s.head[b];
+            s.head[b] = (int8_t)i;
+            int j;
+            for (j = n; j > 0; j--) {
+                int r = s.act[j - 1];
+                if (bits(nc[r]) <= ci) break;
+                if (j == TR_MAXK) {
+                    if (nb == 1) s.head[0] = (int8_t)r;   /* act[32] aliases bucket 0 */
+                } else {
+                    s.act[j] = (int8_t)r;
+                }
+            }
+            if (j < K) s.act[j] = (int8_t)i;
+            if (n < K) n++;
+            continue;
+        }
+        if (tcost <= ci) continue;
+        int tp = n;
+        if (t < TR_LINK_FOREIGN)
+            for (tp = 0; tp < n; tp++)
+                if (s.act[tp] == t) break;
+        int j;
+        if (tp == n) {
+            for (j = n; j > 0; j--) {
+                int r = s.act[j - 1];
+                if (bits(nc[r]) <= ci) break;
+                if (j < K) s.act[j] = (int8_t)r;
+            }
+            if (j < K) s.act[j] = (int8_t)i;
+            if (n < K) n++;
+            continue;
+        }
+        for (j = tp; j + 1 < n; j++) s.act[j] = s.act[j + 1];
+        n--;
+        for (j = n; j > 0; j--) {
+            int r = s.act[j - 1];
+            if (bits(nc[r]) <= ci) break;
+            if (j < K) s.act[j] = (int8_t)r;
+        }
+        if (j < K) s.act[j] = (int8_t)i;
+        n++;
+    }
+    for (int i = 0; i < 64; i++) s.rank[i] = 255;
+    for (int k = 0; k < n; k++) s.rank[s.act[k]] = (uint8_t)k;
+    return n;
+}
+
+} // namespace
+
+template <int P, int HW>
+__global__ __launch_bounds__(64) void tr_run_jobs(tr_params prm, tr_job *jobs, tr_ctx *ctxs)
+{
+    __shared__ TrLds<P, HW> s;
+    const int lane = threadIdx.x;
+    tr_job *jb = &jobs[blockIdx.x];
+    const tr_ctx *c0 = &ctxs[jb->ctx_in];
+    const tr_ctx *snap = jb->snap >= 0 ? &ctxs[jb->snap] : nullptr;
+    const int K = prm.K, L = prm.L;
+    const uint32_t mask = prm.mask;
+    const bool groups_ok = (prm.B <= 8) && !prm.force_slow;
+    int err = 0;
+
+    /* ---- load the context ---- */
+    int num_cands = c0->num_cands, pos = c0->pos, pending = c0->pending, idx = c0->idx;
+    const int draining = c0->draining;
+    {
+        const int cb = idx;
+        for (int i = lane; i < num_cands * P; i += 64) {
+            const int k = i / P, q = i % P;
+            s.st[cb][k][q] = jb->seed ? jb->seed_st[q] : c0->st[k][q];
+        }
+        for (int i = lane; i < num_cands * HW; i += 64)
+            s.hist[cb][i / HW][i % HW] = c0->hist[i / HW][i % HW];
+        if (lane < num_cands) {
+            s.cost[cb][lane] = jb->seed ? 0.0 : c0->cost[lane];
+            s.path[cb][lane] = c0->path[lane];
+            s.next[cb][lane] = c0->next[lane];
+        }
+        for (int b = 0; b < 2; b++) {
+            int8_t l = c0->link[b][lane];
+            if (jb->copy) {
+                if (l >= TR_LINK_FOREIGN) err = 2;
+                else if (l >= 0) l = (int8_t)(l + TR_LINK_FOREIGN);
+            }
+            s.link[b][lane] = l;
+            s.npath[b][lane] = c0->node_path[b][lane];
+            s.ncost[b][lane] = c0->node_cost[b][lane];
+        }
+        for (int i = lane; i < 256; i += 64) s.gtab[i] = 0;
+        for (int i = lane; i < 128; i += 64) s.btab[i] = 0;
+    }
+    wsync();
+
+    const double a0 = prm.a[0];
+    const u64 lowerm = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const double *in = jb->in;
+    const int64_t n_in = jb->n_in, keep_from = jb->keep_from;
+    u64 acc = 0;
+    int nacc = 0;
+    int64_t kept = 0, slow_steps = 0;
+    double xv = 0.0;
+
+    for (int64_t t = 0; t < n_in; t++) {
+        const int tl = (int)(t & 63);
+        if (tl == 0) {
+            const int64_t ti = t + lane;
+            xv = ti < n_in ? in[ti] : 0.0;
+        }
+        const double x = rl_d(xv, tl);
+
+        const int cb = idx, nb = idx ^ 1;
+        const int n = num_cands, m = 2 * n;
+        const bool live = lane < m;
+        const int k = lane >> 1, br = lane & 1;
+        int next_pos = pos + 1;
+        if (next_pos == L) next_pos = 0;
+
+        /* ---- 1. expand ---- */
+        double sp[P], d[P];
+#pragma unroll
+        for (int q = 0; q < P; q++) sp[q] = s.st[cb][k][q];
+        const double pcost = s.cost[cb][k];
+        const uint32_t ppath = s.path[cb][k];
+        const int pnext = s.next[cb][k];
+        const int nnew = (int)((s.hist[cb][k][next_pos >> 6] >> (next_pos & 63)) & 1ull);
+
+        d[0] = sp[0] - prm.g[0] * sp[1] + x - 0.0;
+        double v = x + prm.a[0] * d[0];
+#pragma unroll
+        for (int q = 1; q < P - 1; q++) {
+            d[q] = sp[q] + sp[q - 1] - prm.g[q] * sp[q + 1];
+            v = v + prm.a[q] * d[q];
+        }
+        d[P - 1] = sp[P - 1] + sp[P - 2];
+        v = v + prm.a[P - 1] * d[P - 1];
+        d[0] = d[0] + (br ? -1.0 : 1.0);
+        if (prm.state_limit > 0.0) {
+            const double lim = prm.state_limit;
+#pragma unroll
+            for (int q = 0; q < P; q++) {
+                if (d[q] > lim) d[q] = lim;
+                else if (d[q] < -lim) d[q] = -lim;
+            }
+        }
+        const double e = v + (br ? -a0 : a0);
+        const double cost = pcost + e * e;
+        const uint32_t path = ((ppath << 1) | (uint32_t)br) & mask;
+        const int cnext = pnext;
+        if (ballot(live && !__builtin_isfinite(cost))) err = 1;
+
+        if (live) {
+            s.npath[nb][lane] = path;
+            s.ncost[nb][lane] = cost;
+            s.key[lane] = cost;
+        }
+        wsync();
+
+        /* ---- 2. prune ---- */
+        const u64 ALL = (m == 64) ? ~0ull : ((1ull << m) - 1ull);
+        /* LT bit j: child j has a strictly smaller metric (bits >= m are
+         * don't-care; every use masks them off) */
+        const u64 LT = (m <= 16) ? lt_sweep<16>(s.key, cost)
+                     : (m <= 32) ? lt_sweep<32>(s.key, cost) : lt_sweep<64>(s.key, cost);
+
+        /* equal metrics: strict ranks must be distinct */
+        const int rall = __popcll(LT & ALL);
+        if (live) s.rank[rall] = (uint8_t)lane;
+        wsync();
+        bool slow = !groups_ok || (ballot(live && s.rank[rall] != lane) != 0);
+
+        const u64 N1 = ballot(live && cnext);
+        int nsurv = 0, myrank = 255;
+        if (!slow) {
+            const int min0 = ff1(ballot(live && (LT & ALL) == 0));
+            const int v1 = __popcll(N1);
+            const int maj = (v1 > m - v1) ? 1 : 0;
+            int mn = min0;
+            if ((int)((N1 >> min0) & 1) != maj) {
+                const u64 MJ = maj ? N1 : (ALL & ~N1);
+                const u64 bmm = ballot(live && ((MJ >> lane) & 1) && (LT & MJ) == 0);
+                if (bmm) {
+                    const int bm = ff1(bmm);
+                    if (rl_d(cost, bm) < rl_d(cost, min0) * 1.1) mn = bm;
+                }
+            }
+            const int M = (int)((N1 >> mn) & 1);
+            const u64 CAND = M ? N1 : (ALL & ~N1);
+            const bool cand = (CAND >> lane) & 1;
+
+            /* same-path groups among candidates (paths are < 256 here) */
+            if (cand) atomicOr(&s.gtab[path], 1ull << lane);
+            wsync();
+            const u64 G = cand ? s.gtab[path] : 0ull;
+            wsync();
+            if (cand) s.gtab[path] = 0ull;
+
+            /* fixed point of early skip + duplicate-path resolution */
+            const u64 LTc = LT & CAND & lowerm;
+            u64 PASS = 0, DS = 0, KM = 0;
+            int killt = 64, R = 64;
+            bool pass = false, kil = false;
+            u64 KILLERS = 0;
+            int it;
+            for (it = 0; it < 64; it++) {
+                u64 kb = 0;
+                for (u64 km = KM; km; km &= km - 1) {
+                    const int dd = ff1(km);
+                    if (lane > rl_i(killt, dd)) kb |= 1ull << dd;
+                }
+                const int cnt = __popcll(LTc & ~DS & ~kb);
+                pass = cand && cnt < K;
+                const u64 nPASS = ballot(pass);
+                const u64 gp = G & nPASS;
+                R = gp ? ff1(gp) : 64;
+                const bool dup = pass && R != lane;
+                const bool rlt = dup && ((LT >> (R & 63)) & 1);
+                kil = dup && !rlt;
+                const u64 nDS = ballot(rlt);
+                KILLERS = ballot(kil);
+                int nkt = 64;
+                u64 nK = 0;
+                for (u64 kk = KILLERS; kk; kk &= kk - 1) {
+                    const int s_ = ff1(kk);
+                    const int rs = rl_i(R, s_);
+                    if (!((nK >> rs) & 1)) {
+                        nK |= 1ull << rs;
+                        if (lane == rs) nkt = s_;
+                    }
+                }
+                const bool same = nPASS == PASS && nDS == DS && nK == KM &&
+                                  ballot(nkt != killt) == 0;
+                PASS = nPASS;
+                DS = nDS;
+                KM = nK;
+                killt = nkt;
+                if (same) break;
+            }
+            if (it >= 64) slow = true;
+
+            const bool reg = pass && R == lane;
+            const u64 REG = ballot(reg);
+            if (!slow && K == TR_MAXK && nb == 1) {
+                const u64 gk = G & KILLERS;
+                const u64 FIRSTK = ballot(kil && gk && lane == ff1(gk));
+                const u64 INC = REG | (KILLERS & ~FIRSTK);
+                const u64 CL = ballot(reg && mbcnt(INC) >= K);
+                if (CL) {
+                    const int fc = ff1(CL);
+                    if (ballot(pass && (path & 127u) == 0 && lane > fc)) slow = true;
+                }
+            }
+            if (!slow) {
+                /* hash links of the new registrations: previous registrant
+                 * of the same bucket, in index order */
+                if (reg) atomicOr(&s.btab[path & 127u], 1ull << lane);
+                wsync();
+                if (reg) {
+                    const u64 bm = s.btab[path & 127u] & lowerm;
+                    s.link[nb][lane] = bm ? (int8_t)(63 - __builtin_clzll(bm)) : (int8_t)-1;
+                }
+                wsync();
+                if (reg) s.btab[path & 127u] = 0ull;
+                const u64 F = CAND & ~DS & ~KM;
+                const int rk = __popcll(LT & F);
+                myrank = (((F >> lane) & 1) && rk < K) ? rk : 255;
+                const int nf = __popcll(F);
+                nsurv = nf < K ? nf : K;
+            }
+        }
+        if (slow) {
+            slow_steps++;
+            if (live) s.nnext[lane] = (uint8_t)cnext;
+            wsync();
+            if (lane == 0) {
+                int e2 = 0;
+                s.scal[0] = literal_prune<P, HW>(s, nb, m, K, snap, &e2);
+                s.scal[1] = e2;
+            }
+            wsync();
+            nsurv = s.scal[0];
+            if (s.scal[1]) err = s.scal[1];
+            myrank = live ? s.rank[lane] : 255;
+        }
+
+        /* ---- 3. commit ---- */
+        const bool surv = live && myrank < nsurv;
+        const u64 B0 = ballot(surv && myrank == 0);
+        const int r0 = ff1(B0);
+        const int outbit = (int)((N1 >> r0) & 1);
+        const double minc = rl_d(cost, r0);
+        if (surv) {
+#pragma unroll
+            for (int q = 0; q < P; q++) s.st[nb][myrank][q] = d[q];
+            const double nc = cost - minc;
+            s.cost[nb][myrank] = nc;
+            s.ncost[nb][lane] = nc;
+            s.path[nb][myrank] = path;
+            s.next[nb][myrank] = (uint8_t)nnew;
+            const int pw = pos >> 6;
+            const u64 pbit = 1ull << (pos & 63);
+#pragma unroll
+            for (int w = 0; w < HW; w++) {
+                u64 hv = s.hist[cb][k][w];
+                if (w == pw) hv = (path & 1u) ? (hv | pbit) : (hv & ~pbit);
+                s.hist[nb][myrank][w] = hv;
+            }
+        }
+        wsync();
+        num_cands = nsurv;
+        pos = next_pos;
+        idx = nb;
+
+        /* output bookkeeping (src/trellis.c:1013-1040) */
+        if (pending < L) {
+            pending++;
+        } else if (t >= keep_from) {
+            acc |= (u64)outbit << nacc;
+            if (++nacc == 64) {
+                if (lane == 0) jb->out[kept >> 6] = acc;
+                kept += 64;
+                acc = 0;
+                nacc = 0;
+            }
+        }
+    }
+    if (nacc) {
+        if (lane == 0) jb->out[kept >> 6] = acc;
+        kept += nacc;
+    }
+
+    /* ---- store the final context ---- */
+    if (jb->ctx_out >= 0) {
+        tr_ctx *co = &ctxs[jb->ctx_out];
+        const int cb = idx;
+        for (int i = lane; i < num_cands * P; i += 64) co->st[i / P][i % P] = s.st[cb][i / P][i % P];
+        for (int i = lane; i < num_cands * HW; i += 64) co->hist[i / HW][i % HW] = s.hist[cb][i / HW][i % HW];
+        if (lane < num_cands) {
+            co->cost[lane] = s.cost[cb][lane];
+            co->path[lane] = s.path[cb][lane];
+            co->next[lane] = s.next[cb][lane];
+        }
+        bool fr = false;
+        for (int b = 0; b < 2; b++) {
+            co->link[b][lane] = s.link[b][lane];
+            co->node_path[b][lane] = s.npath[b][lane];
+            co->node_cost[b][lane] = s.ncost[b][lane];
+            fr |= s.link[b][lane] >= TR_LINK_FOREIGN;
+        }
+        const bool anyf = ballot(fr) != 0;
+        if (lane == 0) {
+            co->num_cands = num_cands;
+            co->pos = pos;
+            co->pending = pending;
+            co->idx = idx;
+            co->draining = draining;
+            co->has_foreign = anyf ? 1 : 0;
+            co->err = err;
+        }
+    }
+    if (lane == 0) {
+        jb->kept = kept;
+        jb->slow_steps = slow_steps;
+        jb->err = err;
+    }
+}
+
+#define TR_INST(P, HW) \
+    template __global__ void tr_run_jobs<P, HW>(tr_params, tr_job *, tr_ctx *);
+#define TR_INST_P(P) TR_INST(P, 1) TR_INST(P, 2) TR_INST(P, 4) TR_INST(P, 8) TR_INST(P, 16) TR_INST(P, 32)
+TR_INST_P(4)
+TR_INST_P(5)
+TR_INST_P(6)
+TR_INST_P(7)
+TR_INST_P(8)
+
+/* Host-side launcher table (P in 4..8, HW in {1,2,4,8,16,32}). */
+extern "C" int tr_launch_jobs(const tr_params *prm, tr_job *d_jobs, int njobs,
+                              tr_ctx *d_ctxs, hipStream_t stream)
+{
+    if (njobs <= 0) return 0;
+    const int P = prm->P;
+    const int hw_need = (prm->L + 63) / 64;
+    int HW = 1;
+    while (HW < hw_need) HW *= 2;
+    if (P < 4 || P > 8 || HW > 32) return -1;
+    dim3 grid((unsigned)njobs), block(64);
+#define TR_CASE(PP, HH) \
+    if (P == PP && HW == HH) { \
+        hipLaunchKernelGGL((tr_run_jobs<PP, HH>), grid, block, 0, stream, *prm, d_jobs, d_ctxs); \
+        return hipGetLastError() == hipSuccess ? 0 : -1; }
+#define TR_CASE_P(PP) TR_CASE(PP, 1) TR_CASE(PP, 2) TR_CASE(PP, 4) TR_CASE(PP, 8) TR_CASE(PP, 16) TR_CASE(PP, 32)
+    TR_CASE_P(4) TR_CASE_P(5) TR_CASE_P(6) TR_CASE_P(7) TR_CASE_P(8)
+    return -1;
+}

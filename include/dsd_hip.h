@@ -1,0 +1,149 @@
+/*
+ * dsd_hip — MI355X (gfx950) backend for the PCM->DSD hot path of
+ * foo_dsd_trellis: half-band FIR upsampling chain, Viterbi trellis SDM, and
+ * Density-Aligned-Stitching (DAS) over parallel segments.
+ *
+ * Plain C ABI (libdsdhip.so).  Conventions follow the reference's GPU seam
+ * (include/gpu_compute.h:1-9): functions return 0 on success and -1 on
+ * failure, on which the caller falls back to its CPU path; host buffers are
+ * caller-owned and are copied synchronously.  Functions with a _dev suffix
+ * take device pointers and enqueue on the given HIP stream instead.
+ *
+ * Each declaration names the reference interface it replaces.
+ */
+#ifndef DSD_HIP_H
+#define DSD_HIP_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------
+ * 1. Backend seam — additions to include/gpu_compute.h
+ * ---------------------------------------------------------------------- */
+
+/* New value for gpu_backend_t (include/gpu_compute.h:24-29). */
+#define GPU_BACKEND_HIP 4
+
+/* Same layout as gpu_info_t (include/gpu_compute.h:31-36). */
+typedef struct {
+    int backend;
+    char device_name[128];
+    size_t vram_mb;
+    bool available;
+} gpu_hip_info_t;
+
+/* replaces gpu_cuda_probe (include/gpu_compute.h:185) */
+bool gpu_hip_probe(void);
+/* replaces gpu_cuda_get_info (include/gpu_compute.h:188) */
+void gpu_hip_get_info(gpu_hip_info_t *info);
+/* replaces gpu_cuda_create (include/gpu_compute.h:191); the returned
+ * context's first field is the backend tag GPU_BACKEND_HIP, as the
+ * dispatcher expects (src/gpu_compute.c:153-157). */
+void *gpu_hip_create(void);
+/* replaces gpu_cuda_destroy (include/gpu_compute.h:194) */
+void gpu_hip_destroy(void *ctx);
+/* replaces gpu_cuda_reset_chunk (include/gpu_compute.h:195) */
+void gpu_hip_reset_chunk(void *ctx);
+
+/* replaces gpu_cuda_fir_setup (include/gpu_compute.h:240-241): every stage
+ * uses `taps` (ntaps, odd).  When the taps are the reference's 63-tap
+ * half-band design (src/fir.c:79-98) rounded to float, the double-precision
+ * design is used, as the CPU fp64 chain does. */
+int gpu_hip_fir_setup(void *ctx, const float *taps, int ntaps, int num_stages, bool upsample);
+/* Exact CPU-chain plan for a rate pair (src/fir.c:523-611, incl. the 127-tap
+ * first stage towards DSD512), with the engine's output gain
+ * (src/engine.c:872-883).  Extension of the seam. */
+int gpu_hip_fir_setup_rates(void *ctx, unsigned fs_in, unsigned fs_out, double gain);
+/* replaces gpu_cuda_fir_chain (include/gpu_compute.h:242-243) */
+int gpu_hip_fir_chain(void *ctx, const float *in, float *out, size_t in_count, size_t *out_count);
+/* replaces gpu_cuda_fir_chain_f64 (include/gpu_compute.h:244-245) */
+int gpu_hip_fir_chain_f64(void *ctx, const float *in, double *out, size_t in_count,
+                          size_t *out_count);
+/* replaces gpu_cuda_fir_batch (include/gpu_compute.h:248-250) */
+int gpu_hip_fir_batch(void *ctx, const float *in_batch, float *out_batch, size_t samples_per_ch,
+                      int num_channels, size_t *out_count_per_ch);
+
+/* replaces gpu_cuda_trellis_setup (include/gpu_compute.h:220-223) */
+int gpu_hip_trellis_setup(void *ctx, int num_cands, int order, int trellis_lat,
+                          const double *ntf_a, const double *ntf_g, double state_limit,
+                          int trellis_depth);
+/* replaces gpu_cuda_trellis (include/gpu_compute.h:229): one channel,
+ * device-resident state persisting between calls; output +-1.0 exactly as
+ * sdm_process_block (src/trellis.c:1013-1040) would produce. */
+int gpu_hip_trellis(void *ctx, const double *in, float *out, size_t count);
+/* replaces gpu_cuda_trellis_das (include/gpu_compute.h:232-233): DAS over
+ * num_channels x count samples (channel-major), all segments in one launch,
+ * same semantics as the reference's CPU DAS (src/dsp_plugin.c:1314-2761). */
+int gpu_hip_trellis_das(void *ctx, const double *in, float *out, size_t count, int num_channels);
+/* DAS plan parameters for gpu_hip_trellis_das (segment length in samples,
+ * 0 = reference auto count; seed mode 1 = seg0-first, 0 = all segments
+ * seeded from the persistent state; reposition quiet boundaries). */
+int gpu_hip_das_config(void *ctx, size_t seg_len, int seg0_first, int reposition);
+
+/* ------------------------------------------------------------------------
+ * 2. NTF tables — include/ntf.h:23-36
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    double a[10];
+    double g[10];
+    int order;
+    unsigned freq;
+    const char *name;
+} dsdhip_ntf_t;
+
+/* replaces ntf_get_filter_count / ntf_get_by_index (include/ntf.h:32-35) */
+int dsdhip_ntf_count(void);
+const dsdhip_ntf_t *dsdhip_ntf_get(int index);
+/* replaces ntf_get_filter (include/ntf.h:23): index or -1 */
+int dsdhip_ntf_lookup(int ntf_id, unsigned rate);
+/* replaces ntf_auto_select (include/ntf.h:26): index or -1 */
+int dsdhip_ntf_auto(unsigned rate);
+
+/* ------------------------------------------------------------------------
+ * 3. Trellis contexts on the device — include/trellis.h:61-118
+ * ---------------------------------------------------------------------- */
+typedef struct dsdhip_trellis dsdhip_trellis_t;
+
+/* Pool of nctx contexts sharing one parameter set; replaces
+ * sdm_context_init (include/trellis.h:62-63) for each of them. */
+dsdhip_trellis_t *dsdhip_trellis_create(int ntf_index, int depth, int cands, int lat,
+                                        double state_limit, int nctx);
+dsdhip_trellis_t *dsdhip_trellis_create_coeffs(const double *a, const double *g, int order,
+                                               int depth, int cands, int lat,
+                                               double state_limit, int nctx);
+void dsdhip_trellis_destroy(dsdhip_trellis_t *t);
+int dsdhip_trellis_set_stream(dsdhip_trellis_t *t, void *hip_stream);
+int dsdhip_trellis_num_contexts(const dsdhip_trellis_t *t);
+/* replaces sdm_context_reset (include/trellis.h:86) */
+int dsdhip_trellis_reset(dsdhip_trellis_t *t, int ctx);
+/* replaces sdm_context_copy_state (include/trellis.h:77) */
+int dsdhip_trellis_copy(dsdhip_trellis_t *t, int dst, int src);
+/* DAS seeding of every survivor (src/dsp_plugin.c:2329-2338) */
+int dsdhip_trellis_seed(dsdhip_trellis_t *t, int ctx, const double *state10);
+/* best survivor's loop-filter state (as read by src/dsp_plugin.c:2251-2253) */
+int dsdhip_trellis_best_state(dsdhip_trellis_t *t, int ctx, double *state10);
+/* replaces sdm_process_block (include/trellis.h:70-71), host buffers */
+int dsdhip_trellis_process(dsdhip_trellis_t *t, int ctx, const double *in, float *out,
+                           size_t count, size_t *out_count);
+/* Batched form, device buffers (see dsdhip_trellis.cpp for field meaning):
+ * one launch, one wave per job; outputs are packed bits (1 = +1.0). */
+int dsdhip_trellis_run_jobs(dsdhip_trellis_t *t, int njobs, const int *ctx, const int *copy_src,
+                            const double *const *d_in, const int64_t *n_in,
+                            const int64_t *keep_from, uint64_t *const *d_out,
+                            const double *seed, int64_t *kept, int64_t *slow_steps,
+                            int *err_out);
+/* Validation switch: decide every step with the literal sequential prune. */
+int dsdhip_trellis_force_literal(dsdhip_trellis_t *t, int on);
+
+/* Library version string. */
+const char *dsdhip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
