@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Benchmark of the PCM->DSD hot path on one MI355X.
+
+Workload (one "step"): 30 s of stereo 44.1 kHz float32 PCM (seeded pink
+noise at -12 dBFS RMS clipped to +-0.5 plus 7 tones 100 Hz..15 kHz at
+-20 dBFS) -> eight 2x half-band FIR stages to DSD256 (11.2896 MHz, gain
+0.354) -> DAS parallel trellis (CLANS-8, K=32, L=64, B=8, 65536-sample
+segments, overlap 4096, all segments seeded from the persistent state,
+quiet-boundary reposition) -> stitched 1-bit streams.  Inputs are resident
+in HBM before the timed region; each step starts from fresh FIR delay lines
+and fresh trellis contexts.
+
+Prints one JSON line: Msamples/s of DSD output over both channels, with
+x-realtime, the trellis kernel's roofline position (HIP-event timed), the
+plain per-channel GPU form on a bounded sample, and the CPU oracle timed on
+this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FS_IN = 44100
+DSD256 = 11289600
+SECONDS = 30
+NCH = 2
+K, L, B = 32, 64, 8
+SEG_LEN = 65536
+FP64_PEAK_TFLOPS = 78.6
+
+
+def flops_per_sample(P, K_):
+    """FP64 operations of one trellis step per active candidate: loop filter
+    (P-1 multiply-subtract-add rows, P multiply-adds for v), the two child
+    metrics and the +-1 state update: K * (5P + 6)."""
+    return K_ * (5 * P + 6)
+
+
+def cpu_baseline(x_seg, fi, threads, target_s):
+    """Oracle trellis segments (K=32, L=64) on `threads` host threads."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    O = oracle_lib.oracle()
+    keep = SEG_LEN
+    # calibrate one segment on one thread
+    t0 = time.perf_counter()
+    O.ctx(fi, B, K, L).process(x_seg)
+    per = time.perf_counter() - t0
+    per_thread = max(1, int(round(target_s / per)))
+    done = []
+
+    def work():
+        n = 0
+        for _ in range(per_thread):
+            out = O.ctx(fi, B, K, L).process(x_seg)
+            n += min(keep, out.size)
+        done.append(n)
+
+    ts = [threading.Thread(target=work) for _ in range(threads)]
+    t0 = time.perf_counter()
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    wall = time.perf_counter() - t0
+    total = sum(done)
+    return {"value": round(total / wall / 1e6, 4), "unit": "Msamples/s", "cores": threads,
+            "kind": "port",
+            "sample": "%d threads x %d trellis segments of %d inputs (%d kept outputs each), "
+                      "CLANS-8 K=32 L=64 B=8, oracle C restatement, %.1f s wall"
+                      % (threads, per_thread, x_seg.size, keep, wall)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=1.0,
+                    help="per-thread CPU oracle work for the cpu_baseline leg (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--plain-seconds", type=float, default=0.05,
+                    help="audio seconds per channel for the plain per-channel GPU form (0: skip)")
+    ap.add_argument("--seg-len", type=int, default=SEG_LEN)
+    ap.add_argument("--seg0-first", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus %d is not supported "
+              "(the path would shard whole channels/tracks across GPUs)" % args.gpus)
+        sys.exit(2)
+
+    import torch
+    from foo_dsd_trellis_amd import das, fir, signals, trellis
+
+    dev = torch.device("cuda:0")
+    n_in = FS_IN * SECONDS
+    F = n_in * (DSD256 // FS_IN)
+    x = signals.pink_multitone(n_in, fs=FS_IN, nch=NCH, seed=2)
+    d_x = torch.from_numpy(x).to(dev)
+    d_f = torch.empty((NCH, F), dtype=torch.float64, device=dev)
+    words = F // 64 + 1
+    d_bits = torch.zeros((NCH, words), dtype=torch.int64, device=dev)
+
+    chain = fir.FirChain(FS_IN, DSD256, fir.PCM_FIR_GAIN, NCH)
+    fi = trellis.ntf_auto(DSD256)
+    P = trellis.ntf_row(fi)["order"]
+    D = das.Das(fi, B, K, L, NCH, seg_len=args.seg_len, seg0_first=args.seg0_first, reposition=True)
+    D.timing(True)
+
+    def step():
+        chain.reset()
+        D.reset()
+        chain.process_dev(d_x.data_ptr(), n_in, n_in, d_f.data_ptr(), F)
+        return D.run_dev(d_f.data_ptr(), F, F, d_bits.data_ptr(), words)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    D.timing_read(clear=True)
+    t0 = time.perf_counter()
+    n_out = 0
+    for _ in range(args.steps):
+        n_out = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    k_ms, k_launches, k_samples = D.timing_read(clear=True)
+    ms_step = (t1 - t0) * 1000.0 / args.steps
+    total_out = NCH * n_out
+    value = total_out / (ms_step / 1000.0) / 1e6
+    xrt = SECONDS / (ms_step / 1000.0)
+    nseg, slow, _ = D.last_info(0)
+
+    fps = flops_per_sample(P, K)
+    achieved = (k_samples * fps) / (k_ms / 1000.0) / 1e12 if k_ms > 0 else 0.0
+    roof = {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 6), "traffic": None,
+            "kernel": "tr_run_jobs<8,1>", "avg_launch_ms": round(k_ms / max(1, k_launches), 3),
+            "flops_per_input_sample": fps, "inputs_per_launch": k_samples // max(1, k_launches)}
+
+    plain = None
+    if args.plain_seconds > 0:
+        npl = int(args.plain_seconds * DSD256)
+        xs = d_f[:, :npl].cpu().numpy()
+        pool = trellis.SdmPool(fi, B, K, L, 0.0, NCH)
+        d_in = torch.from_numpy(np.ascontiguousarray(xs)).to(dev)
+        d_o = torch.zeros((NCH, npl // 64 + 2), dtype=torch.int64, device=dev)
+        torch.cuda.synchronize()
+        tp0 = time.perf_counter()
+        kept, _ = pool.run_jobs(list(range(NCH)), [d_in[c].data_ptr() for c in range(NCH)],
+                                [npl] * NCH, [d_o[c].data_ptr() for c in range(NCH)])
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - tp0
+        plain = {"value": round(float(sum(kept)) / tp / 1e6, 4), "unit": "Msamples/s",
+                 "sample": "sequential trellis, one context per channel, %.3f s of DSD256 audio "
+                           "per channel (%d samples)" % (args.plain_seconds, npl)}
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        thr = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        seg = d_f[0, SEG_LEN:SEG_LEN + SEG_LEN + 2 * 64 * L].cpu().numpy()
+        cpu = cpu_baseline(np.ascontiguousarray(seg), fi, thr, args.cpu_seconds)
+
+    line = {
+        "metric": "Trellis SDM Msamples/s & xrealtime at DSD256 K=32, batched vs plain",
+        "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "x_realtime": round(xrt, 3),
+        "config": {"workload": "PCM 44.1k stereo 30 s -> 8x2 half-band FIR -> DSD256 DAS trellis",
+                   "fs_out": DSD256, "seconds": SECONDS, "channels": NCH, "K": K, "L": L, "B": B,
+                   "ntf": trellis.ntf_row(fi)["name"], "seg_len": args.seg_len,
+                   "overlap": 64 * L, "segments_per_channel": nseg,
+                   "seed_mode": "seg0-first" if args.seg0_first else "all-parallel",
+                   "literal_prune_steps": slow, "out_samples_per_channel": n_out},
+        "roofline": roof,
+        "plain_gpu": plain,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -43,6 +43,41 @@ _SIGS = {
     "dsdhip_trellis_run_jobs": ([_vp, C.c_int, _ip, _ip, C.POINTER(_vp), _i64p, _i64p,
                                  C.POINTER(_vp), _dp, _i64p, _i64p, _ip], C.c_int),
     "dsdhip_trellis_force_literal": ([_vp, C.c_int], C.c_int),
+    "dsdhip_fir_plan": ([C.c_uint, C.c_uint, _ip], C.c_int),
+    "dsdhip_halfband_taps": ([_dp, C.c_int], None),
+    "dsdhip_fir_create": ([C.c_uint, C.c_uint, C.c_double, C.c_int], _vp),
+    "dsdhip_fir_destroy": ([_vp], None),
+    "dsdhip_fir_set_stream": ([_vp, _vp], C.c_int),
+    "dsdhip_fir_num_stages": ([_vp], C.c_int),
+    "dsdhip_fir_reset": ([_vp], C.c_int),
+    "dsdhip_fir_process_dev": ([_vp, _vp, C.c_int64, C.c_size_t, _vp, C.c_int64], C.c_int),
+    "dsdhip_fir_process": ([_vp, _fp, C.c_size_t, _dp], C.c_int),
+    "dsdhip_das_create": ([C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_size_t,
+                           C.c_int, C.c_int, C.c_int], _vp),
+    "dsdhip_das_destroy": ([_vp], None),
+    "dsdhip_das_set_stream": ([_vp, _vp], C.c_int),
+    "dsdhip_das_force_literal": ([_vp, C.c_int], C.c_int),
+    "dsdhip_das_plan": ([_vp, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], C.c_int),
+    "dsdhip_das_run_dev": ([_vp, _vp, C.c_int64, C.c_size_t, _vp, C.c_int64, C.POINTER(C.c_size_t)], C.c_int),
+    "dsdhip_das_run": ([_vp, _dp, C.c_size_t, _fp, C.POINTER(C.c_size_t)], C.c_int),
+    "dsdhip_das_last_info": ([_vp, _ip, _i64p, _ip, C.c_int], C.c_int),
+    "dsdhip_trellis_timing": ([_vp, C.c_int], C.c_int),
+    "dsdhip_trellis_timing_read": ([_vp, _dp, _i64p, _i64p, C.c_int], C.c_int),
+    "dsdhip_das_reset": ([_vp], C.c_int),
+    "dsdhip_das_timing": ([_vp, C.c_int], C.c_int),
+    "dsdhip_das_timing_read": ([_vp, _dp, _i64p, _i64p, C.c_int], C.c_int),
+    "gpu_hip_create": ([], _vp),
+    "gpu_hip_destroy": ([_vp], None),
+    "gpu_hip_reset_chunk": ([_vp], None),
+    "gpu_hip_fir_setup": ([_vp, _fp, C.c_int, C.c_int, C.c_bool], C.c_int),
+    "gpu_hip_fir_setup_rates": ([_vp, C.c_uint, C.c_uint, C.c_double], C.c_int),
+    "gpu_hip_fir_chain": ([_vp, _fp, _fp, C.c_size_t, C.POINTER(C.c_size_t)], C.c_int),
+    "gpu_hip_fir_chain_f64": ([_vp, _fp, _dp, C.c_size_t, C.POINTER(C.c_size_t)], C.c_int),
+    "gpu_hip_fir_batch": ([_vp, _fp, _fp, C.c_size_t, C.c_int, C.POINTER(C.c_size_t)], C.c_int),
+    "gpu_hip_trellis_setup": ([_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_int], C.c_int),
+    "gpu_hip_trellis": ([_vp, _dp, _fp, C.c_size_t], C.c_int),
+    "gpu_hip_trellis_das": ([_vp, _dp, _fp, C.c_size_t, C.c_int], C.c_int),
+    "gpu_hip_das_config": ([_vp, C.c_size_t, C.c_int, C.c_int], C.c_int),
 }
 
 _lib = None

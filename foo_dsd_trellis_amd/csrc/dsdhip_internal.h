@@ -14,6 +14,13 @@ extern "C" {
 /* Selects the device on first use; 0 on success. */
 int dsdhip_ensure_device(void);
 
+int dsdhip_trellis_get(dsdhip_trellis_t *t, int i, tr_ctx *out);
+int dsdhip_trellis_put(dsdhip_trellis_t *t, int i, const tr_ctx *in);
+int dsdhip_trellis_clone(dsdhip_trellis_t *t, int dst, int src);
+int dsdhip_trellis_detach(dsdhip_trellis_t *t, int i);
+int dsdhip_trellis_copy_detached(dsdhip_trellis_t *t, int dst, int src);
+int dsdhip_trellis_params(const dsdhip_trellis_t *t, tr_params *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,213 @@
+/*
+ * The gpu_compute.h backend seam for HIP (include/gpu_compute.h:22-59,
+ * 185-233): an opaque context whose first field is the backend tag, holding
+ * the FIR chain, a one-context trellis and a DAS pipeline.  Host buffers in,
+ * host buffers out; 0 / -1 like the reference's backends.
+ */
+#include <hip/hip_runtime.h>
+#include <stdlib.h>
+#include <string.h>
+#include "dsdhip_internal.h"
+
+#define GPU_MIN_SAMPLES 8192   /* include/gpu_compute.h:40 */
+
+struct gpu_hip_context {
+    int backend;   /* GPU_BACKEND_HIP, read by the dispatcher (src/gpu_compute.c:153-157) */
+    dsdhip_fir_t *fir;
+    int fir_nch;
+    unsigned fs_in, fs_out;
+    double fir_gain;
+    /* trellis setup */
+    int have_tr;
+    double a[10], g[10];
+    int order, cands, lat, depth;
+    double state_limit;
+    dsdhip_trellis_t *tr;
+    dsdhip_das_t *das;
+    int das_nch;
+    size_t das_seg_len;
+    int das_seg0_first, das_reposition;
+    /* uniform-taps FIR setup (gpu_hip_fir_setup) */
+    double taps[127];
+    int ntaps, nstages;
+};
+
+extern "C" {
+
+void dsdhip_halfband_taps(double *h, int ntaps);
+
+void *gpu_hip_create(void)
+{
+    if (dsdhip_ensure_device() != 0) return NULL;
+    gpu_hip_context *c = (gpu_hip_context *)calloc(1, sizeof(gpu_hip_context));
+    c->backend = GPU_BACKEND_HIP;
+    c->das_seg_len = 65536;
+    c->das_seg0_first = 0;
+    c->das_reposition = 1;
+    return c;
+}
+
+void gpu_hip_destroy(void *ctx)
+{
+    gpu_hip_context *c = (gpu_hip_context *)ctx;
+    if (!c) return;
+    dsdhip_fir_destroy(c->fir);
+    dsdhip_trellis_destroy(c->tr);
+    dsdhip_das_destroy(c->das);
+    free(c);
+}
+
+void gpu_hip_reset_chunk(void *ctx) { (void)ctx; /* no per-chunk counters on this backend */ }
+
+int gpu_hip_fir_setup(void *ctx, const float *taps, int ntaps, int num_stages, bool upsample)
+{
+    gpu_hip_context *c = (gpu_hip_context *)ctx;
+    if (!c || !taps || !upsample || (ntaps != 63 && ntaps != 127) || num_stages < 1 || num_stages > 9)
+        return -1;
+    double ref[127];
+    dsdhip_halfband_taps(ref, ntaps);
+    int same = 1;
+    for (int i = 0; i < ntaps; i++) if ((float)ref[i] != taps[i]) { same = 0; break; }
+    for (int i = 0; i < ntaps; i++) c->taps[i] = same ? ref[i] : (double)taps[i];
+    c->ntaps = ntaps;
+    c->nstages = num_stages;
+    dsdhip_fir_destroy(c->fir);
+    c->fir = NULL;
+    c->fs_in = c->fs_out = 0;
+    c->fir_gain = 1.0;
+    return 0;
+}
+
+int gpu_hip_fir_setup_rates(void *ctx, unsigned fs_in, unsigned fs_out, double gain)
+{
+    gpu_hip_context *c = (gpu_hip_context *)ctx;
+    if (!c) return -1;
+    int nt[9];
+    if (dsdhip_fir_plan(fs_in, fs_out, nt) < 1) return -1;
+    dsdhip_fir_destroy(c->fir);
+    c->fir = NULL;
+    c->fs_in = fs_in;
+    c->fs_out = fs_out;
+    c->fir_gain = gain;
+    c->ntaps = 0;
+    return 0;
+}
+
+static int ensure_fir(gpu_hip_context *c, int nch)
+{
+    if (c->fir && c->fir_nch == nch) return 0;
+    dsdhip_fir_destroy(c->fir);
+    if (c->fs_in) c->fir = dsdhip_fir_create(c->fs_in, c->fs_out, c->fir_gain, nch);
+    else if (c->ntaps) c->fir = dsdhip_fir_create_taps(c->taps, c->ntaps, c->nstages, 1.0, nch);
+    else c->fir = NULL;
+    c->fir_nch = nch;
+    return c->fir ? 0 : -1;
+}
+
+int gpu_hip_fir_chain_f64(void *ctx, const float *in, double *out, size_t in_count, size_t *out_count)
+{
+    gpu_hip_context *c = (gpu_hip_context *)ctx;
+    if (!c || !in || !out || in_count < GPU_MIN_SAMPLES) return -1;
+    if (ensure_fir(c, 1) != 0) return -1;
+    if (dsdhip_fir_process(c->fir, in, in_count, out) != 0) return -1;
+    if (out_count) *out_count = in_count << dsdhip_fir_num_stages(c->fir);
+    return 0;
+}
+
+int gpu_hip_fir_chain(void *ctx, const float *in, float *out, size_t in_count, size_t *out_count)
+{
+    gpu_hip_context *c = (gpu_hip_context *)ctx;
+    if (!c || !in || !out || in_count < GPU_MIN_SAMPLES) return -1;
+    if (ensure_fir(c, 1) != 0) return -1;
+    const size_t n = in_count << dsdhip_fir_num_stages(c->fir);
+    double *tmp = (double *)malloc(sizeof(double) * n);
+    if (!tmp) return -1;
+    int rc = dsdhip_fir_process(c->fir, in, in_count, tmp);
+    if (rc == 0) {
+        for (size_t i = 0; i < n; i++) out[i] = (float)tmp[i];
+        if (out_count) *out_count = n;
+    }
+    free(tmp);
+    return rc;
+}
+
+int gpu_hip_fir_batch(void *ctx, const float *in_batch, float *out_batch, size_t samples_per_ch,
+                      int num_channels, size_t *out_count_per_ch)
+{
+    gpu_hip_context *c = (gpu_hip_context *)ctx;
+    if (!c || !in_batch || !out_batch || num_channels < 1 || samples_per_ch < GPU_MIN_SAMPLES) return -1;
+    if (ensure_fir(c, num_channels) != 0) return -1;
+    const size_t n = samples_per_ch << dsdhip_fir_num_stages(c->fir);
+    double *tmp = (double *)malloc(sizeof(double) * n * (size_t)num_channels);
+    if (!tmp) return -1;
+    int rc = dsdhip_fir_process(c->fir, in_batch, samples_per_ch, tmp);
+    if (rc == 0) {
+        for (size_t i = 0; i < n * (size_t)num_channels; i++) out_batch[i] = (float)tmp[i];
+        if (out_count_per_ch) *out_count_per_ch = n;
+    }
+    free(tmp);
+    return rc;
+}
+
+int gpu_hip_trellis_setup(void *ctx, int num_cands, int order, int trellis_lat, const double *ntf_a,
+                          const double *ntf_g, double state_limit, int trellis_depth)
+{
+    gpu_hip_context *c = (gpu_hip_context *)ctx;
+    if (!c || !ntf_a || !ntf_g || order < 4 || order > 8) return -1;
+    dsdhip_trellis_destroy(c->tr);
+    c->tr = dsdhip_trellis_create_coeffs(ntf_a, ntf_g, order, trellis_depth, num_cands, trellis_lat,
+                                         state_limit, 1);
+    if (!c->tr) return -1;
+    memset(c->a, 0, sizeof(c->a));
+    memset(c->g, 0, sizeof(c->g));
+    memcpy(c->a, ntf_a, sizeof(double) * (size_t)order);
+    memcpy(c->g, ntf_g, sizeof(double) * (size_t)order);
+    c->order = order;
+    c->cands = num_cands;
+    c->lat = trellis_lat;
+    c->depth = trellis_depth;
+    c->state_limit = state_limit;
+    c->have_tr = 1;
+    dsdhip_das_destroy(c->das);
+    c->das = NULL;
+    return 0;
+}
+
+int gpu_hip_trellis(void *ctx, const double *in, float *out, size_t count)
+{
+    gpu_hip_context *c = (gpu_hip_context *)ctx;
+    if (!c || !c->tr || !in || !out) return -1;
+    size_t n = 0;
+    return dsdhip_trellis_process(c->tr, 0, in, out, count, &n);
+}
+
+int gpu_hip_das_config(void *ctx, size_t seg_len, int seg0_first, int reposition)
+{
+    gpu_hip_context *c = (gpu_hip_context *)ctx;
+    if (!c) return -1;
+    c->das_seg_len = seg_len;
+    c->das_seg0_first = seg0_first;
+    c->das_reposition = reposition;
+    dsdhip_das_destroy(c->das);
+    c->das = NULL;
+    return 0;
+}
+
+int gpu_hip_trellis_das(void *ctx, const double *in, float *out, size_t count, int num_channels)
+{
+    gpu_hip_context *c = (gpu_hip_context *)ctx;
+    if (!c || !c->have_tr || !in || !out || num_channels < 1) return -1;
+    if (!c->das || c->das_nch != num_channels) {
+        dsdhip_das_destroy(c->das);
+        c->das = dsdhip_das_create_coeffs(c->a, c->g, c->order, c->depth, c->cands, c->lat,
+                                          c->state_limit, num_channels, c->das_seg_len,
+                                          c->das_seg_len ? 0 : 2, c->das_seg0_first,
+                                          c->das_reposition);
+        c->das_nch = num_channels;
+        if (!c->das) return -1;
+    }
+    size_t n = 0;
+    return dsdhip_das_run(c->das, in, count, out, &n);
+}
+
+} /* extern "C" */

@@ -25,6 +25,12 @@ struct dsdhip_trellis {
     tr_job *d_jobs;
     int jobs_cap;
     hipStream_t stream;
+    /* optional HIP-event timing of the trellis launches */
+    int timing;
+    hipEvent_t ev0, ev1;
+    double kernel_ms;        /* accumulated */
+    int64_t kernel_launches;
+    int64_t samples;         /* inputs consumed by timed launches */
 };
 
 static void ctx_fresh(tr_ctx *c)
@@ -62,6 +68,10 @@ dsdhip_trellis_t *dsdhip_trellis_create_coeffs(const double *a, const double *g,
     t->d_jobs = NULL;
     t->jobs_cap = 0;
     t->stream = 0;
+    t->timing = 0;
+    t->kernel_ms = 0.0;
+    t->kernel_launches = 0;
+    t->samples = 0;
     if (hipMalloc(&t->d_ctx, sizeof(tr_ctx) * (size_t)nctx) != hipSuccess) {
         delete t;
         return NULL;
@@ -78,9 +88,48 @@ dsdhip_trellis_t *dsdhip_trellis_create(int ntf_index, int depth, int cands, int
     return dsdhip_trellis_create_coeffs(f->a, f->g, f->order, depth, cands, lat, state_limit, nctx);
 }
 
+/* Enables HIP-event timing of the trellis launches on the pool's stream;
+ * the accumulated kernel time, launch count and consumed input samples are
+ * read (and cleared) with dsdhip_trellis_timing_read. */
+int dsdhip_trellis_timing(dsdhip_trellis_t *t, int on)
+{
+    if (!t) return -1;
+    if (on && !t->timing) {
+        if (hipEventCreate(&t->ev0) != hipSuccess || hipEventCreate(&t->ev1) != hipSuccess) return -1;
+    }
+    if (!on && t->timing) {
+        hipEventDestroy(t->ev0);
+        hipEventDestroy(t->ev1);
+    }
+    t->timing = on ? 1 : 0;
+    t->kernel_ms = 0.0;
+    t->kernel_launches = 0;
+    t->samples = 0;
+    return 0;
+}
+
+int dsdhip_trellis_timing_read(dsdhip_trellis_t *t, double *ms, int64_t *launches, int64_t *samples,
+                               int clear)
+{
+    if (!t) return -1;
+    if (ms) *ms = t->kernel_ms;
+    if (launches) *launches = t->kernel_launches;
+    if (samples) *samples = t->samples;
+    if (clear) {
+        t->kernel_ms = 0.0;
+        t->kernel_launches = 0;
+        t->samples = 0;
+    }
+    return 0;
+}
+
 void dsdhip_trellis_destroy(dsdhip_trellis_t *t)
 {
     if (!t) return;
+    if (t->timing) {
+        hipEventDestroy(t->ev0);
+        hipEventDestroy(t->ev1);
+    }
     hipFree(t->d_ctx);
     if (t->d_jobs) hipFree(t->d_jobs);
     delete t;
@@ -149,6 +198,49 @@ int dsdhip_trellis_copy(dsdhip_trellis_t *t, int dst, int src)
     t->snap_of[dst] = src;
     return dsdhip_trellis_put(t, dst, &c);
 }
+
+/* Byte-exact duplicate of a context (no copy_state semantics). */
+int dsdhip_trellis_clone(dsdhip_trellis_t *t, int dst, int src)
+{
+    if (!t || dst < 0 || src < 0 || dst >= t->nctx || src >= t->nctx) return -1;
+    if (dst == src) return 0;
+    t->snap_of[dst] = t->snap_of[src];
+    return hipMemcpyAsync(&t->d_ctx[dst], &t->d_ctx[src], sizeof(tr_ctx), hipMemcpyDeviceToDevice,
+                          t->stream) == hipSuccess ? 0 : -1;
+}
+
+/* Drops every path-hash link of a context (they are only followed by
+ * lookups that walk past a bucket overwritten at K=32). */
+int dsdhip_trellis_detach(dsdhip_trellis_t *t, int i)
+{
+    tr_ctx c;
+    if (dsdhip_trellis_get(t, i, &c) != 0) return -1;
+    memset(c.link, 0xff, sizeof(c.link));
+    c.has_foreign = 0;
+    t->snap_of[i] = -1;
+    return dsdhip_trellis_put(t, i, &c);
+}
+
+/* copy_state followed by detach, in one round trip. */
+int dsdhip_trellis_copy_detached(dsdhip_trellis_t *t, int dst, int src)
+{
+    if (!t || dst < 0 || src < 0 || dst >= t->nctx || src >= t->nctx) return -1;
+    tr_ctx c;
+    if (dsdhip_trellis_get(t, src, &c) != 0) return -1;
+    memset(c.link, 0xff, sizeof(c.link));
+    c.has_foreign = 0;
+    t->snap_of[dst] = -1;
+    return dsdhip_trellis_put(t, dst, &c);
+}
+
+int dsdhip_trellis_params(const dsdhip_trellis_t *t, tr_params *out)
+{
+    if (!t || !out) return -1;
+    *out = t->prm;
+    return 0;
+}
+
+hipStream_t dsdhip_trellis_stream(const dsdhip_trellis_t *t) { return t ? t->stream : 0; }
 
 int dsdhip_trellis_seed(dsdhip_trellis_t *t, int i, const double *st)
 {
@@ -221,11 +313,19 @@ int dsdhip_trellis_run_jobs(dsdhip_trellis_t *t, int njobs, const int *ctx, cons
     if (hipMemcpyAsync(t->d_jobs, jobs.data(), sizeof(tr_job) * (size_t)njobs,
                        hipMemcpyHostToDevice, t->stream) != hipSuccess)
         return -1;
+    if (t->timing) hipEventRecord(t->ev0, t->stream);
     if (tr_launch_jobs(&t->prm, t->d_jobs, njobs, t->d_ctx, t->stream) != 0) return -1;
+    if (t->timing) hipEventRecord(t->ev1, t->stream);
     if (hipMemcpyAsync(jobs.data(), t->d_jobs, sizeof(tr_job) * (size_t)njobs,
                        hipMemcpyDeviceToHost, t->stream) != hipSuccess)
         return -1;
     if (hipStreamSynchronize(t->stream) != hipSuccess) return -1;
+    if (t->timing) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t->ev0, t->ev1) == hipSuccess) t->kernel_ms += ms;
+        t->kernel_launches++;
+        for (int j = 0; j < njobs; j++) t->samples += n_in[j];
+    }
     int err = 0;
     for (int j = 0; j < njobs; j++) {
         if (kept) kept[j] = jobs[(size_t)j].kept;

@@ -1,0 +1,80 @@
+"""Host-side mirror of the reference's DAS parallel trellis
+(src/dsp_plugin.c:1314-2761) on the HIP backend."""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import DsdHipError, check, lib
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+
+
+class Das:
+    """nch persistent trellis contexts + DAS plan (segment length or count,
+    seed mode seg0-first / all-parallel, boundary reposition)."""
+
+    def __init__(self, ntf_index, depth, cands, lat, nch, seg_len=65536, nseg=0,
+                 seg0_first=False, reposition=True, state_limit=0.0):
+        self.h = lib().dsdhip_das_create(ntf_index, depth, cands, lat, state_limit, nch, seg_len,
+                                         nseg, 1 if seg0_first else 0, 1 if reposition else 0)
+        if not self.h:
+            raise DsdHipError("dsdhip_das_create rejected the parameters")
+        self.nch = nch
+
+    def close(self):
+        if self.h:
+            lib().dsdhip_das_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, s):
+        check(lib().dsdhip_das_set_stream(self.h, s), "das set_stream")
+
+    def force_literal(self, on=True):
+        check(lib().dsdhip_das_force_literal(self.h, 1 if on else 0), "force_literal")
+
+    def nseg(self, F):
+        return lib().dsdhip_das_plan(self.h, F, None, None)
+
+    def run(self, x):
+        """x: (nch, F) float64 host -> ((nch, F) float32 +-1, assembled length)."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(self.nch, -1)
+        F = x.shape[1]
+        out = np.zeros((self.nch, F), np.float32)
+        cnt = C.c_size_t(0)
+        check(lib().dsdhip_das_run(self.h, x.ctypes.data_as(_dp), F, out.ctypes.data_as(_fp),
+                                   C.byref(cnt)), "das run")
+        return out, cnt.value
+
+    def run_dev(self, d_in, in_stride, F, d_out, out_words):
+        cnt = C.c_size_t(0)
+        check(lib().dsdhip_das_run_dev(self.h, d_in, in_stride, F, d_out, out_words, C.byref(cnt)),
+              "das run_dev")
+        return cnt.value
+
+    def reset(self):
+        check(lib().dsdhip_das_reset(self.h), "das reset")
+
+    def timing(self, on=True):
+        check(lib().dsdhip_das_timing(self.h, 1 if on else 0), "das timing")
+
+    def timing_read(self, clear=True):
+        ms = C.c_double(0)
+        nl = C.c_int64(0)
+        ns = C.c_int64(0)
+        check(lib().dsdhip_das_timing_read(self.h, C.byref(ms), C.byref(nl), C.byref(ns),
+                                           1 if clear else 0), "timing_read")
+        return ms.value, nl.value, ns.value
+
+    def last_info(self, max_pos=0):
+        ns = C.c_int(0)
+        sl = C.c_int64(0)
+        pos = (C.c_int * max(1, max_pos))()
+        check(lib().dsdhip_das_last_info(self.h, C.byref(ns), C.byref(sl), pos, max_pos), "last_info")
+        return ns.value, sl.value, list(pos[:max_pos])

@@ -139,6 +139,74 @@ int dsdhip_trellis_run_jobs(dsdhip_trellis_t *t, int njobs, const int *ctx, cons
                             int *err_out);
 /* Validation switch: decide every step with the literal sequential prune. */
 int dsdhip_trellis_force_literal(dsdhip_trellis_t *t, int on);
+/* HIP-event timing of trellis launches on the pool's stream (no reference
+ * counterpart; the reference times phases with QueryPerformanceCounter,
+ * src/dsp_plugin.c:90-94). */
+int dsdhip_trellis_timing(dsdhip_trellis_t *t, int on);
+int dsdhip_trellis_timing_read(dsdhip_trellis_t *t, double *ms, int64_t *launches, int64_t *samples,
+                               int clear);
+
+/* ------------------------------------------------------------------------
+ * 4. FIR upsampling chain — include/fir.h:22-79
+ * ---------------------------------------------------------------------- */
+typedef struct dsdhip_fir dsdhip_fir_t;
+
+/* Stage plan for fs_in -> fs_out (2^n, n <= 9): per-stage tap count into
+ * ntaps[]; returns the stage count or -1 (src/fir.c:523-611, :131-150). */
+int dsdhip_fir_plan(unsigned fs_in, unsigned fs_out, int *ntaps);
+/* The reference's half-band design (src/fir.c:79-98, beta 12). */
+void dsdhip_halfband_taps(double *h, int ntaps);
+/* replaces fir_chain_init_ex(..., use_fp64=true) (include/fir.h:61-62) for
+ * nch channels, output multiplied by gain (src/engine.c:872-883). */
+dsdhip_fir_t *dsdhip_fir_create(unsigned fs_in, unsigned fs_out, double gain, int nch);
+/* uniform taps on every stage (gpu_compute.h gpu_fir_setup semantics) */
+dsdhip_fir_t *dsdhip_fir_create_taps(const double *taps, int ntaps, int nstages, double gain, int nch);
+/* replaces fir_chain_free (include/fir.h:79) */
+void dsdhip_fir_destroy(dsdhip_fir_t *f);
+int dsdhip_fir_set_stream(dsdhip_fir_t *f, void *hip_stream);
+int dsdhip_fir_num_stages(const dsdhip_fir_t *f);
+/* replaces fir_chain_reset (include/fir.h:76): zero delay lines */
+int dsdhip_fir_reset(dsdhip_fir_t *f);
+/* replaces fir_chain_process_d (include/fir.h:70-72) for nch channels;
+ * delay lines persist between calls.  Device buffers, stream-ordered. */
+int dsdhip_fir_process_dev(dsdhip_fir_t *f, const float *d_in, int64_t in_stride, size_t n,
+                           double *d_out, int64_t out_stride);
+/* Host buffers: in nch x n floats, out nch x n*2^stages doubles. */
+int dsdhip_fir_process(dsdhip_fir_t *f, const float *in, size_t n, double *out);
+
+/* ------------------------------------------------------------------------
+ * 5. DAS — src/dsp_plugin.c:1314-2761 (parallel trellis + stitching)
+ * ---------------------------------------------------------------------- */
+typedef struct dsdhip_das dsdhip_das_t;
+
+/* nch persistent trellis contexts plus segment contexts grown on demand.
+ * seg_len > 0: segments of seg_len samples (last takes the rest);
+ * seg_len == 0: nseg equal parts shrunk until each is >= 4 overlaps
+ * (:1348-1368).  Overlap = 64 * lat (:1347). */
+dsdhip_das_t *dsdhip_das_create(int ntf_index, int depth, int cands, int lat, double state_limit,
+                                int nch, size_t seg_len, int nseg, int seg0_first, int reposition);
+dsdhip_das_t *dsdhip_das_create_coeffs(const double *a, const double *g, int order, int depth,
+                                       int cands, int lat, double state_limit, int nch,
+                                       size_t seg_len, int nseg, int seg0_first, int reposition);
+void dsdhip_das_destroy(dsdhip_das_t *d);
+int dsdhip_das_set_stream(dsdhip_das_t *d, void *hip_stream);
+int dsdhip_das_force_literal(dsdhip_das_t *d, int on);
+/* Segment count (and nominal starts/sizes if given) for F samples. */
+int dsdhip_das_plan(const dsdhip_das_t *d, size_t F, size_t *start, size_t *size);
+/* One chunk: d_in nch x F doubles (row stride in_stride); d_out nch rows
+ * of out_words packed-bit words (bit = 1 for +1.0). */
+int dsdhip_das_run_dev(dsdhip_das_t *d, const double *d_in, int64_t in_stride, size_t F,
+                       unsigned long long *d_out, int64_t out_words, size_t *out_count);
+/* Host buffers: in nch x F doubles, out nch x F floats (+-1.0). */
+int dsdhip_das_run(dsdhip_das_t *d, const double *in, size_t F, float *out, size_t *out_count);
+/* Fresh persistent contexts (sdm_context_reset, include/trellis.h:86). */
+int dsdhip_das_reset(dsdhip_das_t *d);
+/* HIP-event timing of the segment trellis launches (see dsdhip_trellis_timing). */
+int dsdhip_das_timing(dsdhip_das_t *d, int on);
+int dsdhip_das_timing_read(dsdhip_das_t *d, double *ms, int64_t *launches, int64_t *samples, int clear);
+/* Segment count, literal-prune steps and stitch positions of the last run. */
+int dsdhip_das_last_info(const dsdhip_das_t *d, int *nseg, int64_t *slow_steps, int *stitch_pos,
+                         int max_pos);
 
 /* Library version string. */
 const char *dsdhip_version(void);

@@ -109,3 +109,65 @@ long ora_hash0_slot(const void *cv)
             return b * 256 + (long)(t - &c->node[b][0]);
     return -2;   /* points into another context */
 }
+
+/* DAS over nch channels of F samples (channel-major) on fresh contexts of
+ * the given parameters; returns assembled length, fills out (nch*F floats)
+ * and stitch positions (nseg ints).  plan_out receives nom_start/nom_size
+ * (2*nseg) if non-NULL.  seg_len 0 => nseg equal parts. */
+long ora_das(int filter_index, int depth, int cands, int lat, const double *in, int nch,
+             size_t F, size_t seg_len, int nseg, int seg0_first, int reposition,
+             float *out, int *stitch_pos, long *plan_out, int max_seg)
+{
+    or_das_plan_t p;
+    int S = or_das_plan_init(&p, F, seg_len, nseg, (size_t)64 * (size_t)lat, seg0_first, reposition);
+    if (S > max_seg) { or_das_plan_free(&p); return -2; }
+    if (reposition) or_das_reposition(&p, in, F);
+    or_ctx *pc = (or_ctx *)calloc((size_t)nch, sizeof(or_ctx));
+    for (int ch = 0; ch < nch; ch++)
+        or_sdm_init(&pc[ch], or_ntf_get(filter_index), depth, cands, lat);
+    size_t n = or_das_run(&p, pc, nch, in, F, out, stitch_pos);
+    if (plan_out)
+        for (int s = 0; s < S; s++) {
+            plan_out[2 * s] = (long)p.nom_start[s];
+            plan_out[2 * s + 1] = (long)p.nom_size[s];
+        }
+    free(pc);
+    or_das_plan_free(&p);
+    return (long)n;
+}
+
+int ora_das_nseg(size_t F, size_t seg_len, int nseg, int lat)
+{
+    or_das_plan_t p;
+    int S = or_das_plan_init(&p, F, seg_len, nseg, (size_t)64 * (size_t)lat, 1, 0);
+    or_das_plan_free(&p);
+    return S;
+}
+
+int ora_stitch(const float *prev, const float *cur, int ovl, int lat, double *q, int *nc)
+{
+    return or_das_stitch(prev, cur, ovl, lat, q, nc);
+}
+
+size_t ora_fir_chain(unsigned fs_in, unsigned fs_out, const float *in, size_t n, double gain,
+                     double *out)
+{
+    return or_fir_chain(fs_in, fs_out, in, n, gain, out);
+}
+
+size_t ora_fir_chain_ld(unsigned fs_in, unsigned fs_out, const float *in, size_t n, double gain,
+                        double *out)
+{
+    size_t cnt = n;
+    int nt[OR_FIR_MAX_STAGES];
+    int st = or_fir_plan(fs_in, fs_out, nt);
+    if (st < 0) return 0;
+    cnt = n << st;
+    long double *tmp = (long double *)malloc(cnt * sizeof(long double));
+    size_t r = or_fir_chain_ld(fs_in, fs_out, in, n, gain, tmp);
+    for (size_t i = 0; i < r; i++) out[i] = (double)tmp[i];
+    free(tmp);
+    return r;
+}
+
+void ora_halfband(double *h, int ntaps) { or_design_halfband(h, ntaps, 12.0); }

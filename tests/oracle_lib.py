@@ -151,3 +151,57 @@ def pack_bits(pm1):
     """+-1.0 float stream -> packed little-endian bits (1 for +1)."""
     b = (np.asarray(pm1) > 0).astype(np.uint8)
     return np.packbits(b, bitorder="little")
+
+
+def das(lib_, filter_index, depth, cands, lat, x, seg_len=0, nseg=2, seg0_first=1, reposition=1,
+        max_seg=20000):
+    """Oracle DAS over x (nch, F) float64 -> (out (nch, F) float32, n, stitch, plan)."""
+    L = lib_.lib
+    L.ora_das.restype = C.c_long
+    L.ora_das.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, C.c_int, C.c_size_t, C.c_size_t,
+                          C.c_int, C.c_int, C.c_int, _fp, C.POINTER(C.c_int), C.POINTER(C.c_long),
+                          C.c_int]
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    nch, F = x.shape
+    out = np.zeros((nch, F), np.float32)
+    sp = np.zeros(max_seg, np.int32)
+    plan = np.zeros(2 * max_seg, np.int64)
+    n = L.ora_das(filter_index, depth, cands, lat, _ptr(x, _dp), nch, F, seg_len, nseg, seg0_first,
+                  reposition, _ptr(out, _fp), sp.ctypes.data_as(C.POINTER(C.c_int)),
+                  plan.ctypes.data_as(C.POINTER(C.c_long)), max_seg)
+    if n < 0:
+        raise ValueError("oracle DAS rejected the plan")
+    L.ora_das_nseg.argtypes = [C.c_size_t, C.c_size_t, C.c_int, C.c_int]
+    S = L.ora_das_nseg(F, seg_len, nseg, lat)
+    return out, int(n), sp[:S], plan[:2 * S].reshape(S, 2)
+
+
+def fir_chain(lib_, fs_in, fs_out, x, gain=1.0, long_double=False):
+    """Oracle FIR chain of one channel x (float32) -> float64."""
+    L = lib_.lib
+    fn = L.ora_fir_chain_ld if long_double else L.ora_fir_chain
+    fn.restype = C.c_size_t
+    fn.argtypes = [C.c_uint, C.c_uint, _fp, C.c_size_t, C.c_double, _dp]
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    r = fs_out // fs_in
+    out = np.zeros(x.size * r)
+    n = fn(fs_in, fs_out, _ptr(x, _fp), x.size, gain, _ptr(out, _dp))
+    return out[:n]
+
+
+def halfband(lib_, ntaps):
+    h = np.zeros(ntaps)
+    lib_.lib.ora_halfband.argtypes = [_dp, C.c_int]
+    lib_.lib.ora_halfband(_ptr(h, _dp), ntaps)
+    return h
+
+
+def stitch(lib_, prev, cur, lat):
+    L = lib_.lib
+    L.ora_stitch.argtypes = [_fp, _fp, C.c_int, C.c_int, _dp, C.POINTER(C.c_int)]
+    prev = np.ascontiguousarray(prev, np.float32)
+    cur = np.ascontiguousarray(cur, np.float32)
+    q = C.c_double(0)
+    nc = C.c_int(0)
+    pos = L.ora_stitch(_ptr(prev, _fp), _ptr(cur, _fp), prev.size, lat, C.byref(q), C.byref(nc))
+    return pos, q.value, nc.value

@@ -1,0 +1,143 @@
+"""GPU parity: HIP trellis / FIR / DAS against the CPU oracle on seeded
+inputs (bit-exact for DSD bits, identical doubles for the FIR)."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from foo_dsd_trellis_amd import das as D
+from foo_dsd_trellis_amd import fir as FIR
+from foo_dsd_trellis_amd import signals as SG
+from foo_dsd_trellis_amd import trellis as T
+
+pytestmark = pytest.mark.gpu
+
+
+def _x(kind, n, rate, seed):
+    rng = np.random.default_rng(seed)
+    t = np.arange(n) / rate
+    if kind == "sine":
+        return 0.5012 * np.sin(2 * np.pi * 997 * 16 * t) + 1e-5 * rng.standard_normal(n)
+    if kind == "pink":
+        return np.clip(0.18 * SG.pink(n, rng), -0.5, 0.5)
+    if kind == "loud":   # drives bucket-0 lookups after the act[32] write at K=32
+        return 0.45 * np.sin(2 * np.pi * 997 * 16 * t) + 0.05 * rng.standard_normal(n)
+    raise ValueError(kind)
+
+
+TRELLIS_CASES = [
+    # K, L, B, rate, n, kind
+    (8, 16, 8, 2822400, 30000, "sine"),
+    (32, 64, 8, 11289600, 30000, "pink"),
+    (32, 64, 8, 11289600, 30000, "loud"),
+    (16, 32, 8, 5644800, 20000, "pink"),
+    (32, 100, 16, 11289600, 8000, "pink"),
+    (4, 8, 4, 2822400, 8000, "sine"),
+    (1, 1, 1, 2822400, 4000, "sine"),
+    (32, 200, 8, 22579200, 6000, "loud"),
+]
+
+
+@pytest.mark.parametrize("K,L,B,rate,n,kind", TRELLIS_CASES)
+def test_trellis_bit_exact(gpu, oracle, K, L, B, rate, n, kind):
+    x = _x(kind, n, rate, K * 131 + L)
+    fi = oracle.ntf_auto(rate)
+    ref = oracle.ctx(fi, B, K, L).process(x)
+    p = T.SdmPool(fi, B, K, L, 0.0, 1)
+    y = p.process(0, x)
+    assert len(y) == len(ref)
+    bad = np.nonzero(y != ref)[0]
+    assert bad.size == 0, "first mismatch at %d of %d" % (bad[0], len(ref))
+
+
+@pytest.mark.parametrize("K,L,B,rate,n,kind", TRELLIS_CASES[:4])
+def test_trellis_literal_path_bit_exact(gpu, oracle, K, L, B, rate, n, kind):
+    """Every step decided by the kernel's literal prune gives the same bits."""
+    x = _x(kind, n // 2, rate, K * 7 + L)
+    fi = oracle.ntf_auto(rate)
+    ref = oracle.ctx(fi, B, K, L).process(x)
+    p = T.SdmPool(fi, B, K, L, 0.0, 1)
+    p.force_literal(True)
+    y = p.process(0, x)
+    assert np.array_equal(y, ref)
+
+
+def test_trellis_state_limit_and_continuation(gpu, oracle):
+    """Block-wise processing continues exactly (persistent device state)."""
+    fi = oracle.ntf_auto(11289600)
+    x = _x("loud", 12000, 11289600, 5)
+    ref = oracle.ctx(fi, 8, 32, 64, 2.5).process(x)
+    p = T.SdmPool(fi, 8, 32, 64, 2.5, 1)
+    ys = [p.process(0, x[a:b]) for a, b in [(0, 33), (33, 5000), (5000, 5001), (5001, 12000)]]
+    assert np.array_equal(np.concatenate(ys), ref)
+
+
+def test_trellis_copy_and_seed(gpu, oracle):
+    """copy_state + seeding as the DAS dispatch does, then continue."""
+    fi = oracle.ntf_auto(11289600)
+    x = _x("pink", 20000, 11289600, 9)
+    o_src = oracle.ctx(fi, 8, 32, 64)
+    o_src.process(x[:8000])
+    o_dst = oracle.ctx(fi, 8, 32, 64)
+    o_dst.copy_from(o_src)
+    st = o_src.best_state() * 0.5
+    o_dst.seed(st)
+    ref = o_dst.process(x[8000:])
+    p = T.SdmPool(fi, 8, 32, 64, 0.0, 2)
+    p.process(0, x[:8000])
+    p.copy(1, 0)
+    p.seed(1, st)
+    y = p.process(1, x[8000:])
+    assert np.array_equal(y, ref)
+
+
+@pytest.mark.parametrize("fs_in,fs_out,n,gain", [
+    (44100, 2822400, 44100, FIR.PCM_FIR_GAIN),
+    (44100, 11289600, 6000, FIR.PCM_FIR_GAIN),
+    (48000, 24576000, 3000, 1.0),
+])
+def test_fir_matches_oracle(gpu, oracle, fs_in, fs_out, n, gain):
+    x = SG.sine_noise(n, fs=fs_in, nch=2, seed=fs_out % 97)
+    f = FIR.FirChain(fs_in, fs_out, gain, nch=2)
+    y = f.process(x)
+    for c in range(2):
+        ref = ol.fir_chain(oracle, fs_in, fs_out, x[c], gain)
+        assert np.array_equal(y[c], ref), np.max(np.abs(y[c] - ref))
+        truth = ol.fir_chain(oracle, fs_in, fs_out, x[c], gain, long_double=True)
+        assert np.max(np.abs(y[c] - truth)) < 1e-12
+
+
+def test_fir_streaming_equals_one_shot(gpu):
+    x = SG.sine_noise(40000, nch=2, seed=3)
+    f1 = FIR.FirChain(44100, 2822400, 1.0, 2)
+    whole = f1.process(x)
+    f2 = FIR.FirChain(44100, 2822400, 1.0, 2)
+    parts = [f2.process(x[:, a:b]) for a, b in [(0, 10), (10, 9000), (9000, 40000)]]
+    assert np.array_equal(np.concatenate(parts, axis=1), whole)
+
+
+DAS_CASES = [
+    # K, L, B, rate, F, seg_len, nseg, seg0_first, kind
+    (8, 16, 8, 2822400, 60000, 0, 8, 1, "sine"),
+    (8, 16, 8, 2822400, 60000, 0, 8, 0, "sine"),
+    (32, 64, 8, 11289600, 120000, 16384, 0, 0, "pink"),
+    (32, 64, 8, 11289600, 120000, 16384, 0, 1, "pink"),
+    (16, 32, 8, 5644800, 90000, 0, 5, 1, "loud"),
+]
+
+
+@pytest.mark.parametrize("K,L,B,rate,F,seg_len,nseg,seg0,kind", DAS_CASES)
+def test_das_bit_exact(gpu, oracle, K, L, B, rate, F, seg_len, nseg, seg0, kind):
+    nch = 2
+    x = np.stack([_x(kind, F, rate, 11 + c) for c in range(nch)])
+    fi = oracle.ntf_auto(rate)
+    ref, n, sp, plan = ol.das(oracle, fi, B, K, L, x, seg_len=seg_len, nseg=nseg,
+                              seg0_first=seg0, reposition=1)
+    d = D.Das(fi, B, K, L, nch, seg_len=seg_len, nseg=nseg, seg0_first=bool(seg0), reposition=True)
+    y, m = d.run(x)
+    S, slow, pos = d.last_info(len(sp))
+    assert S == len(sp)
+    assert m == n
+    assert list(pos[1:]) == list(sp[1:]) or S == 1
+    for c in range(nch):
+        bad = np.nonzero(y[c, :n] != ref[c, :n])[0]
+        assert bad.size == 0, "ch%d first mismatch %d" % (c, bad[0])
