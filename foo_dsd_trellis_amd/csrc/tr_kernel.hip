@@ -48,20 +48,37 @@ __device__ __forceinline__ int mbcnt(u64 m)
                                           __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
-__device__ __forceinline__ void wsync() { __syncthreads(); }
+/* The workgroup is one wave: its LDS instructions execute in issue order,
+ * so cross-lane exchange through LDS only needs the compiler not to move
+ * memory operations across this point. */
+__device__ __forceinline__ void wsync() { __asm__ volatile("" ::: "memory"); }
+
+/* acc * 2 + (k < c): the compare result feeds the add-with-carry directly
+ * (two VALU instructions per element). */
+__device__ __forceinline__ unsigned shl_add_lt(unsigned acc, double k, double c)
+{
+    unsigned r;
+    __asm__("v_cmp_lt_f64_e32 vcc, %1, %2\n\t"
+            "v_addc_co_u32_e32 %0, vcc, %3, %3, vcc"
+            : "=v"(r)
+            : "v"(k), "v"(c), "v"(acc)
+            : "vcc");
+    return r;
+}
 
 /* Strict-order mask against the first NJ metrics of the LDS key array
- * (broadcast reads, fully unrolled). */
+ * (broadcast reads, unrolled by 16). */
 template <int NJ>
 __device__ __forceinline__ u64 lt_sweep(const double *key, double c)
 {
     unsigned lo = 0, hi = 0;
-#pragma unroll
-    for (int j = 0; j < NJ; j++) {
-        const bool b = key[j] < c;
-        if (j < 32) lo |= b ? (1u << (j & 31)) : 0u;
-        else hi |= b ? (1u << (j & 31)) : 0u;
+    /* descending j, shift-in: bit j of the word ends at position j % 32 */
+    if (NJ > 32) {
+#pragma unroll 16
+        for (int j = NJ - 1; j >= 32; j--) hi = shl_add_lt(hi, key[j], c);
     }
+#pragma unroll 16
+    for (int j = (NJ < 32 ? NJ : 32) - 1; j >= 0; j--) lo = shl_add_lt(lo, key[j], c);
     return ((u64)hi << 32) | lo;
 }
 
