@@ -85,14 +85,14 @@ __device__ __forceinline__ u64 lt_sweep(const double *key, double c)
 /* LDS working set of one context. */
 template <int P, int HW>
 struct TrLds {
-    double st[2][TR_MAXK][P];
+    double st[2][P][TR_MAXK];      /* [bank][q][k]: lanes of a step read consecutive k */
     double cost[2][TR_MAXK];
     u64 hist[2][TR_MAXK][HW];
     uint32_t path[2][TR_MAXK];
     uint8_t next[2][TR_MAXK];
     double key[64];
-    u64 gtab[256];
-    u64 btab[128];
+    u64 gtab[256 + 64];            /* + one scratch entry per lane */
+    u64 btab[128 + 64];
     uint32_t npath[2][64];
     double ncost[2][64];
     int8_t link[2][64];
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(64) void tr_run_jobs(tr_params prm, tr_job *jobs, t
         const int cb = idx;
         for (int i = lane; i < num_cands * P; i += 64) {
             const int k = i / P, q = i % P;
-            s.st[cb][k][q] = jb->seed ? jb->seed_st[q] : c0->st[k][q];
+            s.st[cb][q][k] = jb->seed ? jb->seed_st[q] : c0->st[k][q];
         }
         for (int i = lane; i < num_cands * HW; i += 64)
             s.hist[cb][i / HW][i % HW] = c0->hist[i / HW][i % HW];
@@ -248,8 +248,8 @@ __global__ __launch_bounds__(64) void tr_run_jobs(tr_params prm, tr_job *jobs, t
             s.npath[b][lane] = c0->node_path[b][lane];
             s.ncost[b][lane] = c0->node_cost[b][lane];
         }
-        for (int i = lane; i < 256; i += 64) s.gtab[i] = 0;
-        for (int i = lane; i < 128; i += 64) s.btab[i] = 0;
+        for (int i = lane; i < 256 + 64; i += 64) s.gtab[i] = 0;
+        for (int i = lane; i < 128 + 64; i += 64) s.btab[i] = 0;
     }
     wsync();
 
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(64) void tr_run_jobs(tr_params prm, tr_job *jobs, t
         /* ---- 1. expand ---- */
         double sp[P], d[P];
 #pragma unroll
-        for (int q = 0; q < P; q++) sp[q] = s.st[cb][k][q];
+        for (int q = 0; q < P; q++) sp[q] = s.st[cb][q][k];
         const double pcost = s.cost[cb][k];
         const uint32_t ppath = s.path[cb][k];
         const int pnext = s.next[cb][k];
@@ -297,11 +297,12 @@ __global__ __launch_bounds__(64) void tr_run_jobs(tr_params prm, tr_job *jobs, t
         v = v + prm.a[P - 1] * d[P - 1];
         d[0] = d[0] + (br ? -1.0 : 1.0);
         if (prm.state_limit > 0.0) {
-            const double lim = prm.state_limit;
+            /* src/trellis.c:179-187, as selects (no per-lane branches) */
+            const double lim = prm.state_limit, nlim = -prm.state_limit;
 #pragma unroll
             for (int q = 0; q < P; q++) {
-                if (d[q] > lim) d[q] = lim;
-                else if (d[q] < -lim) d[q] = -lim;
+                const double v0 = d[q];
+                d[q] = (v0 > lim) ? lim : ((v0 < nlim) ? nlim : v0);
             }
         }
         const double e = v + (br ? -a0 : a0);
@@ -310,10 +311,10 @@ __global__ __launch_bounds__(64) void tr_run_jobs(tr_params prm, tr_job *jobs, t
         const int cnext = pnext;
         if (ballot(live && !__builtin_isfinite(cost))) err = 1;
 
+        s.key[lane] = cost;   /* entries >= m are never counted */
         if (live) {
             s.npath[nb][lane] = path;
             s.ncost[nb][lane] = cost;
-            s.key[lane] = cost;
         }
         wsync();
 
@@ -350,11 +351,12 @@ __global__ __launch_bounds__(64) void tr_run_jobs(tr_params prm, tr_job *jobs, t
             const bool cand = (CAND >> lane) & 1;
 
             /* same-path groups among candidates (paths are < 256 here) */
-            if (cand) atomicOr(&s.gtab[path], 1ull << lane);
+            const int gi = cand ? (int)path : 256 + lane;
+            atomicOr(&s.gtab[gi], 1ull << lane);
             wsync();
-            const u64 G = cand ? s.gtab[path] : 0ull;
+            const u64 G = s.gtab[gi] & (cand ? ~0ull : 0ull);
             wsync();
-            if (cand) s.gtab[path] = 0ull;
+            s.gtab[gi] = 0ull;
 
             /* fixed point of early skip + duplicate-path resolution */
             const u64 LTc = LT & CAND & lowerm;
@@ -414,14 +416,13 @@ __global__ __launch_bounds__(64) void tr_run_jobs(tr_params prm, tr_job *jobs, t
             if (!slow) {
                 /* hash links of the new registrations: previous registrant
                  * of the same bucket, in index order */
-                if (reg) atomicOr(&s.btab[path & 127u], 1ull << lane);
+                const int bi = reg ? (int)(path & 127u) : 128 + lane;
+                atomicOr(&s.btab[bi], 1ull << lane);
                 wsync();
-                if (reg) {
-                    const u64 bm = s.btab[path & 127u] & lowerm;
-                    s.link[nb][lane] = bm ? (int8_t)(63 - __builtin_clzll(bm)) : (int8_t)-1;
-                }
+                const u64 bm = s.btab[bi] & lowerm;
+                if (reg) s.link[nb][lane] = bm ? (int8_t)(63 - __builtin_clzll(bm)) : (int8_t)-1;
                 wsync();
-                if (reg) s.btab[path & 127u] = 0ull;
+                s.btab[bi] = 0ull;
                 const u64 F = CAND & ~DS & ~KM;
                 const int rk = __popcll(LT & F);
                 myrank = (((F >> lane) & 1) && rk < K) ? rk : 255;
@@ -452,7 +453,7 @@ __global__ __launch_bounds__(64) void tr_run_jobs(tr_params prm, tr_job *jobs, t
         const double minc = rl_d(cost, r0);
         if (surv) {
 #pragma unroll
-            for (int q = 0; q < P; q++) s.st[nb][myrank][q] = d[q];
+            for (int q = 0; q < P; q++) s.st[nb][q][myrank] = d[q];
             const double nc = cost - minc;
             s.cost[nb][myrank] = nc;
             s.ncost[nb][lane] = nc;
@@ -494,7 +495,7 @@ __global__ __launch_bounds__(64) void tr_run_jobs(tr_params prm, tr_job *jobs, t
     if (jb->ctx_out >= 0) {
         tr_ctx *co = &ctxs[jb->ctx_out];
         const int cb = idx;
-        for (int i = lane; i < num_cands * P; i += 64) co->st[i / P][i % P] = s.st[cb][i / P][i % P];
+        for (int i = lane; i < num_cands * P; i += 64) co->st[i / P][i % P] = s.st[cb][i % P][i / P];
         for (int i = lane; i < num_cands * HW; i += 64) co->hist[i / HW][i % HW] = s.hist[cb][i / HW][i % HW];
         if (lane < num_cands) {
             co->cost[lane] = s.cost[cb][lane];
