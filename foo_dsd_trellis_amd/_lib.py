@@ -78,6 +78,16 @@ _SIGS = {
     "gpu_hip_trellis": ([_vp, _dp, _fp, C.c_size_t], C.c_int),
     "gpu_hip_trellis_das": ([_vp, _dp, _fp, C.c_size_t, C.c_int], C.c_int),
     "gpu_hip_das_config": ([_vp, C.c_size_t, C.c_int, C.c_int], C.c_int),
+    "dsdhip_lf_checkpoints": ([C.c_size_t, C.c_int], C.c_size_t),
+    "dsdhip_lf_workspace_bytes": ([C.c_int, C.c_size_t, C.c_int, C.c_int, C.c_int], C.c_size_t),
+    "dsdhip_lf_forward": ([_vp, _vp, C.c_int, C.c_double, _vp, C.c_int, C.c_size_t, C.c_int, _vp,
+                           _vp, _vp], C.c_int),
+    "dsdhip_lf_backward": ([_vp, _vp, C.c_int, C.c_double, _vp, C.c_int, C.c_size_t, C.c_int, _vp,
+                            _vp, _vp, _vp, _vp, _vp, C.c_size_t, C.c_int, _vp], C.c_int),
+    "dsdhip_pe_workspace_bytes": ([C.c_int, C.c_size_t, C.c_int], C.c_size_t),
+    "dsdhip_pe_forward": ([_vp, C.c_int, _vp, C.c_int, C.c_size_t, _vp, _vp], C.c_int),
+    "dsdhip_pe_backward": ([_vp, C.c_int, C.c_size_t, C.c_int, _vp, _vp, _vp, C.c_size_t, _vp],
+                           C.c_int),
 }
 
 _lib = None

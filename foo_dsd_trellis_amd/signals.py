@@ -58,3 +58,20 @@ def uneven_tracks(ntracks=16, nch=6, fs=48000, lo_s=5.0, hi_s=120.0, seed=4):
     """Lengths (samples) of the batched workload's tracks, U[5, 120] s."""
     rng = np.random.default_rng(seed)
     return [int(round(s * fs)) for s in rng.uniform(lo_s, hi_s, size=ntracks)]
+
+
+def multitone_noise(nrows, n, fs=5644800, seed=9, dbfs=-10.0, ntones=5, fmin=200.0, fmax=18000.0):
+    """Rows of seeded multitone plus Gaussian noise at `dbfs` RMS (float32):
+    equal-amplitude log-spaced tones carry half the power, noise the rest."""
+    rng = np.random.default_rng(seed)
+    rms = 10.0 ** (dbfs / 20.0)
+    t = np.arange(n) / fs
+    freqs = np.geomspace(fmin, fmax, ntones)
+    amp = np.sqrt(2.0 * 0.5 * rms * rms / ntones)
+    out = np.empty((nrows, n), np.float32)
+    for r in range(nrows):
+        ph = rng.uniform(0, 2 * np.pi, ntones)
+        x = sum(amp * np.sin(2 * np.pi * f * t + ph[i]) for i, f in enumerate(freqs))
+        x = x + np.sqrt(0.5) * rms * rng.standard_normal(n)
+        out[r] = x.astype(np.float32)
+    return out

@@ -208,6 +208,41 @@ int dsdhip_das_timing_read(dsdhip_das_t *d, double *ms, int64_t *launches, int64
 int dsdhip_das_last_info(const dsdhip_das_t *d, int *nseg, int64_t *slow_steps, int *stitch_pos,
                          int max_pos);
 
+/* ======================================================================
+ * 6. Differentiable loop filter and pre-emphasis (training).
+ *    Replaces the per-sample torch loop of training_pipeline/diff_sdm.py:
+ *    61-100 (DiffSDM.forward) and the conv1d of model_pre_sdm.py:44-53
+ *    (FixedPreEmphasis.forward).  All pointers are device memory owned by
+ *    the caller; sequences are time-major [N][B] float64; the NTF
+ *    coefficients a[order], g[order] are device float64.  Nothing here
+ *    allocates; calls are asynchronous on `stream` (a hipStream_t, NULL =
+ *    default stream).  0 = success, -1 = bad arguments or launch failure.
+ * ====================================================================== */
+/* Number of state checkpoints (one every ck_every samples). */
+size_t dsdhip_lf_checkpoints(size_t N, int ck_every);
+/* Workspace bytes of dsdhip_lf_backward processing slab_chunks checkpoint
+ * intervals per slab (<= 0: all of them in one slab). */
+size_t dsdhip_lf_workspace_bytes(int B, size_t N, int order, int ck_every, int slab_chunks);
+/* y = relaxed loop filter of u (quantizer tanh(v * temperature)); writes
+ * the state before every ck_every-th sample to ck[nck][order][B]. */
+int dsdhip_lf_forward(const double *d_a, const double *d_g, int order, double temperature,
+                      const double *d_u, int B, size_t N, int ck_every, double *d_y,
+                      double *d_ck, void *stream);
+/* Adjoint: given gy = dL/dy, writes gu = dL/du and ga, gg = dL/da, dL/dg
+ * (order values each), recomputing states from the checkpoints. */
+int dsdhip_lf_backward(const double *d_a, const double *d_g, int order, double temperature,
+                       const double *d_u, int B, size_t N, int ck_every, const double *d_ck,
+                       const double *d_gy, double *d_gu, double *d_ga, double *d_gg, void *d_ws,
+                       size_t ws_bytes, int slab_chunks, void *stream);
+/* Pre-emphasis: u[n][b] = sum_k w[k] p[b][n-k] (p row-major [B][N] float32,
+ * zero history, 1 <= ntaps <= 128). */
+size_t dsdhip_pe_workspace_bytes(int B, size_t N, int ntaps);
+int dsdhip_pe_forward(const double *d_w, int ntaps, const float *d_p, int B, size_t N,
+                      double *d_u, void *stream);
+/* Tap gradient gw[k] = sum_{b,n} gu[n][b] p[b][n-k]. */
+int dsdhip_pe_backward(const float *d_p, int B, size_t N, int ntaps, const double *d_gu,
+                       double *d_gw, void *d_ws, size_t ws_bytes, void *stream);
+
 /* Library version string. */
 const char *dsdhip_version(void);
 

@@ -157,6 +157,15 @@ int or_das_stitch(const float *prev_ovl, const float *this_ovl, int ovl_len,
                   int lat, double *quality_out, int *ncand_out);
 const double *or_das_qfir(void);
 
+/* Differentiable loop filter and pre-emphasis (or_lf.c); arrays [B][N]. */
+double or_lf_tanh(double z);
+void or_lf_forward(const double *a, const double *g, int P, double T, const double *u, int B,
+                   size_t N, double *y);
+void or_lf_backward(const double *a, const double *g, int P, double T, const double *u, int B,
+                    size_t N, const double *gy, double *gu, double *ga, double *gg);
+void or_pe_forward(const double *w, int T, const float *p, int B, size_t N, double *u);
+void or_pe_backward(int T, const float *p, int B, size_t N, const double *gu, double *gw);
+
 #ifdef __cplusplus
 }
 #endif

@@ -205,3 +205,64 @@ def stitch(lib_, prev, cur, lat):
     nc = C.c_int(0)
     pos = L.ora_stitch(_ptr(prev, _fp), _ptr(cur, _fp), prev.size, lat, C.byref(q), C.byref(nc))
     return pos, q.value, nc.value
+
+
+def lf_forward(lib_, a, g, T, u):
+    """Oracle relaxed loop filter: u (B, N) float64 -> y (B, N)."""
+    L = lib_.lib
+    L.or_lf_forward.argtypes = [_dp, _dp, C.c_int, C.c_double, _dp, C.c_int, C.c_size_t, _dp]
+    a = np.ascontiguousarray(a, np.float64)
+    g = np.ascontiguousarray(g, np.float64)
+    u = np.ascontiguousarray(u, np.float64)
+    y = np.zeros_like(u)
+    L.or_lf_forward(_ptr(a, _dp), _ptr(g, _dp), a.size, T, _ptr(u, _dp), u.shape[0], u.shape[1],
+                    _ptr(y, _dp))
+    return y
+
+
+def lf_backward(lib_, a, g, T, u, gy):
+    """Oracle adjoint: returns (gu (B, N), ga (P,), gg (P,))."""
+    L = lib_.lib
+    L.or_lf_backward.argtypes = [_dp, _dp, C.c_int, C.c_double, _dp, C.c_int, C.c_size_t, _dp, _dp,
+                                 _dp, _dp]
+    a = np.ascontiguousarray(a, np.float64)
+    g = np.ascontiguousarray(g, np.float64)
+    u = np.ascontiguousarray(u, np.float64)
+    gy = np.ascontiguousarray(gy, np.float64)
+    gu = np.zeros_like(u)
+    ga = np.zeros(a.size)
+    gg = np.zeros(a.size)
+    L.or_lf_backward(_ptr(a, _dp), _ptr(g, _dp), a.size, T, _ptr(u, _dp), u.shape[0], u.shape[1],
+                     _ptr(gy, _dp), _ptr(gu, _dp), _ptr(ga, _dp), _ptr(gg, _dp))
+    return gu, ga, gg
+
+
+def pe_forward(lib_, w, p):
+    L = lib_.lib
+    L.or_pe_forward.argtypes = [_dp, C.c_int, _fp, C.c_int, C.c_size_t, _dp]
+    w = np.ascontiguousarray(w, np.float64)
+    p = np.ascontiguousarray(p, np.float32)
+    u = np.zeros(p.shape)
+    L.or_pe_forward(_ptr(w, _dp), w.size, _ptr(p, _fp), p.shape[0], p.shape[1], _ptr(u, _dp))
+    return u
+
+
+def pe_backward(lib_, ntaps, p, gu):
+    L = lib_.lib
+    L.or_pe_backward.argtypes = [C.c_int, _fp, C.c_int, C.c_size_t, _dp, _dp]
+    p = np.ascontiguousarray(p, np.float32)
+    gu = np.ascontiguousarray(gu, np.float64)
+    gw = np.zeros(ntaps)
+    L.or_pe_backward(ntaps, _ptr(p, _fp), p.shape[0], p.shape[1], _ptr(gu, _dp), _ptr(gw, _dp))
+    return gw
+
+
+def inband_weights(N, fs, f_hi=20000.0):
+    """One-sided rFFT weights of the in-band error power (0..f_hi):
+    L = sum_k w_k |E_k|^2 / N^2, w = 2 inside the band except DC/Nyquist (1)."""
+    k = np.arange(N // 2 + 1)
+    w = np.where(k * fs / N <= f_hi, 2.0, 0.0)
+    w[0] = 1.0 if w[0] else 0.0
+    if N % 2 == 0 and w[-1]:
+        w[-1] = 1.0
+    return w

@@ -1,0 +1,212 @@
+"""Pins the differentiable loop-filter oracle (oracle/or_lf.c): forward
+against an independent torch float64 statement of the recurrence in
+training_pipeline/diff_sdm.py:61-100, adjoint against torch float64 autograd
+of that statement, and against central finite differences (h = 1e-3)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle_lib as ol
+
+DSD128 = 5644800
+T_QUANT = 10.0                      # tanh(v / 0.1)
+
+
+def torch_forward(a, g, u, T):
+    """diff_sdm.py's per-sample recurrence, vectorised over the batch."""
+    B, N = u.shape
+    P = a.numel()
+    s = [torch.zeros(B, dtype=torch.float64) for _ in range(P)]
+    ys = []
+    for n in range(N):
+        x = u[:, n]
+        ns = [None] * P
+        ns[0] = s[0] - g[0] * s[1] + x
+        v = x + a[0] * ns[0]
+        for i in range(1, P - 1):
+            ns[i] = s[i] + s[i - 1] - g[i] * s[i + 1]
+            v = v + a[i] * ns[i]
+        ns[P - 1] = s[P - 1] + s[P - 2]
+        v = v + a[P - 1] * ns[P - 1]
+        y = torch.tanh(v * T)
+        ns[0] = ns[0] - y
+        s = ns
+        ys.append(y)
+    return torch.stack(ys, 1)
+
+
+def inband_loss_t(y, r, fs):
+    N = y.shape[1]
+    w = torch.from_numpy(ol.inband_weights(N, fs))
+    E = torch.fft.rfft(y - r, dim=1)
+    return (w * (E.real ** 2 + E.imag ** 2)).sum() / (N * N * y.shape[0])
+
+
+def inband_loss_np(y, r, fs):
+    N = y.shape[1]
+    E = np.fft.rfft(y - r, axis=1)
+    return float((ol.inband_weights(N, fs) * np.abs(E) ** 2).sum() / (N * N * y.shape[0]))
+
+
+def inband_grad_np(y, r, fs):
+    """dL/dy of inband_loss_np: 2/(N^2 B) sum_k w_k Re(E_k e^{+i2pi kn/N}),
+    which is 2/(N B) * irfft(w E / c) with c = 2 inside, 1 at DC/Nyquist."""
+    B, N = y.shape
+    E = np.fft.rfft(y - r, axis=1)
+    w = ol.inband_weights(N, fs)
+    c = np.full(N // 2 + 1, 2.0)
+    c[0] = 1.0
+    if N % 2 == 0:
+        c[-1] = 1.0
+    return 2.0 * np.fft.irfft(w * E / c, N, axis=1) / (N * B)
+
+
+def signal(B, N, seed, fs=DSD128):
+    rng = np.random.default_rng(seed)
+    t = np.arange(N) / fs
+    x = 0.1 * np.sin(2 * np.pi * 1000 * t)[None, :] + 0.1 * np.sin(2 * np.pi * 7000 * t + 1)[None, :]
+    return (x + 0.05 * rng.standard_normal((B, N))).astype(np.float32)
+
+
+def ntf(oracle, rate=DSD128):
+    r = oracle.ntf_get(oracle.ntf_auto(rate))
+    P = r["order"]
+    return r["a"][:P].copy(), r["g"][:P].copy()
+
+
+def test_project_tanh_against_libm(oracle):
+    """The loop filter's tanh (built from correctly rounded operations so
+    host and GPU agree bit for bit) is within 4 ulp of libm tanh."""
+    import ctypes as C
+    f = oracle.lib.or_lf_tanh
+    f.restype = C.c_double
+    f.argtypes = [C.c_double]
+    rng = np.random.default_rng(0)
+    z = np.concatenate([rng.uniform(-25, 25, 20000), rng.uniform(-1e-3, 1e-3, 5000),
+                        rng.uniform(-0.4, 0.4, 5000), [0.0, 1e-300, 19.0, 19.5, 30.0, 1e300]])
+    y = np.array([f(v) for v in z])
+    ref = np.tanh(z)
+    assert np.max(np.abs(y - ref) / np.spacing(np.abs(ref) + 1e-320)) <= 4
+    assert f(-0.0) == 0.0 and np.signbit(f(-0.0))
+    assert np.isnan(f(float("nan"))) and f(float("inf")) == 1.0 and f(-float("inf")) == -1.0
+
+
+# (temperature, samples): at temperature 1 the relaxed loop contracts and
+# long runs compare; at temperature 10 it is chaotic (ulp differences of two
+# tanh implementations reach O(1) after ~100 samples), so only short runs can
+# be compared against a statement that uses torch.tanh.
+REGIMES = [(1.0, 700), (10.0, 20)]
+
+
+@pytest.mark.parametrize("T,N", REGIMES)
+def test_forward_matches_torch_statement(oracle, T, N):
+    a, g = ntf(oracle)
+    u = signal(3, N, 1).astype(np.float64)
+    y = ol.lf_forward(oracle, a, g, T, u)
+    yt = torch_forward(torch.from_numpy(a), torch.from_numpy(g), torch.from_numpy(u), T).numpy()
+    assert np.max(np.abs(y - yt)) < 1e-12
+    assert np.all(np.abs(y) <= 1.0)
+
+
+def test_high_temperature_is_chaotic(oracle):
+    """Documents the regime: a 1e-12 input perturbation reaches O(1) output
+    differences within 400 samples at temperature 10, and decays at 1."""
+    a, g = ntf(oracle)
+    u = signal(1, 2000, 1).astype(np.float64)
+    u2 = u.copy()
+    u2[0, 10] += 1e-12
+    d10 = np.abs(ol.lf_forward(oracle, a, g, 10.0, u) - ol.lf_forward(oracle, a, g, 10.0, u2))
+    d1 = np.abs(ol.lf_forward(oracle, a, g, 1.0, u) - ol.lf_forward(oracle, a, g, 1.0, u2))
+    assert d10[0, :400].max() > 0.1
+    assert d1[0, 1000:].max() < 1e-14
+
+
+def test_loss_gradient_helper():
+    rng = np.random.default_rng(0)
+    y = rng.standard_normal((2, 256))
+    r = rng.standard_normal((2, 256))
+    yt = torch.from_numpy(y).requires_grad_()
+    inband_loss_t(yt, torch.from_numpy(r), 256 * 100.0).backward()
+    assert abs(inband_loss_np(y, r, 25600.0) - float(inband_loss_t(torch.from_numpy(y), torch.from_numpy(r), 25600.0))) < 1e-15
+    assert np.allclose(inband_grad_np(y, r, 25600.0), yt.grad.numpy(), rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("rate,T,N", [(DSD128, 1.0, 600), (2822400, 1.0, 600), (11289600, 1.0, 600),
+                                      (DSD128, 10.0, 20), (11289600, 10.0, 20)])
+def test_adjoint_matches_autograd(oracle, rate, T, N):
+    a, g = ntf(oracle, rate)
+    B, Tn = 2, 63
+    p = signal(B, N, 2, rate)
+    rng = np.random.default_rng(3)
+    w = np.zeros(Tn)
+    w[0] = 1.0
+    w[1:] = 0.01 * rng.standard_normal(Tn - 1)
+    at = torch.from_numpy(a).requires_grad_()
+    gt = torch.from_numpy(g).requires_grad_()
+    wt = torch.from_numpy(w).requires_grad_()
+    pt = torch.from_numpy(p.astype(np.float64))
+    ut = torch.nn.functional.conv1d(torch.nn.functional.pad(pt[:, None, :], (Tn - 1, 0)),
+                                    wt.flip(0)[None, None, :])[:, 0, :]
+    yt = torch_forward(at, gt, ut, T)
+    loss = inband_loss_t(yt, pt, rate)
+    loss.backward()
+
+    u = ol.pe_forward(oracle, w, p)
+    assert np.max(np.abs(u - ut.detach().numpy())) < 1e-14
+    y = ol.lf_forward(oracle, a, g, T, u)
+    gy = inband_grad_np(y, p.astype(np.float64), rate)
+    gu, ga, gg = ol.lf_backward(oracle, a, g, T, u, gy)
+    gw = ol.pe_backward(oracle, Tn, p, gu)
+
+    def rel(x, ref):
+        return np.max(np.abs(x - ref)) / np.max(np.abs(ref))
+    assert rel(ga, at.grad.numpy()) < 1e-9
+    assert rel(gg[:-1], gt.grad.numpy()[:-1]) < 1e-9
+    assert gg[-1] == 0.0 and abs(gt.grad.numpy()[-1]) == 0.0
+    assert rel(gw, wt.grad.numpy()) < 1e-9
+
+
+@pytest.mark.parametrize("T,N,h,tol", [(1.0, 4096, 1e-3, 1e-3), (10.0, 16, 1e-7, 1e-3)])
+def test_adjoint_matches_central_differences(oracle, T, N, h, tol):
+    """Central differences with step h on the taps and relative steps
+    theta*(1 +- h) on the NTF coefficients.  h = 1e-3 at temperature 1; at
+    temperature 10 the loss is not smooth on that scale even over 24
+    samples (d/dtaps[1] = 310 while the h = 1e-3 difference quotient reads
+    0.2), so there the step is 1e-7 over 16 samples."""
+    a, g = ntf(oracle)
+    B, Tn = 2, 15
+    p = signal(B, N, 5)
+    r = p.astype(np.float64)
+    w = np.zeros(Tn)
+    w[0] = 1.0
+    w[1] = -0.007
+
+    def loss(a_, g_, w_):
+        u = ol.pe_forward(oracle, w_, p)
+        return inband_loss_np(ol.lf_forward(oracle, a_, g_, T, u), r, DSD128)
+
+    u = ol.pe_forward(oracle, w, p)
+    y = ol.lf_forward(oracle, a, g, T, u)
+    gu, ga, gg = ol.lf_backward(oracle, a, g, T, u, inband_grad_np(y, r, DSD128))
+    gw = ol.pe_backward(oracle, Tn, p, gu)
+    errs = []
+    for k in (0, 1, 5, 14):
+        wp, wm = w.copy(), w.copy()
+        wp[k] += h
+        wm[k] -= h
+        fd = (loss(a, g, wp) - loss(a, g, wm)) / (2 * h)
+        errs.append(abs(fd - gw[k]) / max(abs(gw).max(), 1e-30))
+    for i in range(len(a)):
+        for arr, grad in ((a, ga), (g, gg)):
+            if arr[i] == 0.0:
+                continue
+            d = h * abs(arr[i])
+            ap, am = arr.copy(), arr.copy()
+            ap[i] += d
+            am[i] -= d
+            if arr is a:
+                fd = (loss(ap, g, w) - loss(am, g, w)) / (2 * d)
+            else:
+                fd = (loss(a, ap, w) - loss(a, am, w)) / (2 * d)
+            errs.append(abs(fd - grad[i]) / max(abs(grad[i]), 1e-30))
+    assert max(errs) < tol, errs
