@@ -126,6 +126,13 @@ extern "C" __global__ __launch_bounds__(256) void das_stitch(
     __shared__ double redmax[DAS_MAXQ + 1][4];
     const int b = blockIdx.x;
     const int ovl = ovl_len[b];
+    if (ovl <= 0) {            /* unused boundary slot of a fixed table */
+        if (threadIdx.x == 0) {
+            o_ncand[b] = 0;
+            o_peak[b] = 0;
+        }
+        return;
+    }
     const int64_t pb = prev_bit[b], cbit = cur_bit[b];
     int hw = hw_lat;
     if (hw > ovl / 2) hw = ovl / 2;

@@ -64,7 +64,7 @@ struct dsdhip_das {
 
 /* 255-tap Kaiser (beta 8) low-pass used to score stitch candidates
  * (src/dsp_plugin.c:2571-2604). */
-static void make_qfir(double *q)
+extern "C" void dsdhip_make_qfir(double *q)
 {
     const int N = 255;
     const double fc = 20000.0 / (5644800.0 / 2.0);
@@ -148,7 +148,7 @@ dsdhip_das_t *dsdhip_das_create_coeffs(const double *a, const double *g, int ord
     d->d_qfir = NULL;
     d->last_S = 0;
     d->last_slow = 0;
-    make_qfir(d->qfir);
+    dsdhip_make_qfir(d->qfir);
     if (dsdhip_ensure_device() != 0 ||
         hipMalloc(&d->d_qfir, sizeof(d->qfir)) != hipSuccess ||
         hipMemcpy(d->d_qfir, d->qfir, sizeof(d->qfir), hipMemcpyHostToDevice) != hipSuccess) {

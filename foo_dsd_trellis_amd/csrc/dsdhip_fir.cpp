@@ -171,11 +171,9 @@ int dsdhip_fir_reset(dsdhip_fir_t *f)
 
 /* d_in: nch rows of n floats (row stride in_stride); d_out: nch rows of
  * n * 2^nst doubles (row stride out_stride). */
-int dsdhip_fir_process_dev(dsdhip_fir_t *f, const float *d_in, int64_t in_stride, size_t n,
-                           double *d_out, int64_t out_stride)
+int dsdhip_fir_reserve(dsdhip_fir_t *f, size_t n)
 {
-    if (!f || !d_in || !d_out) return -1;
-    if (n == 0) return 0;
+    if (!f) return -1;
     const size_t need = (f->nst > 1) ? n * ((size_t)1 << (f->nst - 1)) * (size_t)f->nch : 0;
     if (need > f->buf_cap) {
         for (int b = 0; b < 2; b++) {
@@ -187,6 +185,15 @@ int dsdhip_fir_process_dev(dsdhip_fir_t *f, const float *d_in, int64_t in_stride
             if (hipMalloc(&f->d_buf[b], sizeof(double) * need) != hipSuccess) return -1;
         f->buf_cap = need;
     }
+    return 0;
+}
+
+int dsdhip_fir_process_dev(dsdhip_fir_t *f, const float *d_in, int64_t in_stride, size_t n,
+                           double *d_out, int64_t out_stride)
+{
+    if (!f || !d_in || !d_out) return -1;
+    if (n == 0) return 0;
+    if (dsdhip_fir_reserve(f, n) != 0) return -1;
     const void *src = d_in;
     int src_f32 = 1;
     int64_t src_stride = in_stride;

@@ -20,6 +20,12 @@ int dsdhip_trellis_clone(dsdhip_trellis_t *t, int dst, int src);
 int dsdhip_trellis_detach(dsdhip_trellis_t *t, int i);
 int dsdhip_trellis_copy_detached(dsdhip_trellis_t *t, int dst, int src);
 int dsdhip_trellis_params(const dsdhip_trellis_t *t, tr_params *out);
+void *dsdhip_trellis_context_ptr(dsdhip_trellis_t *t, int i);
+/* 255-tap Kaiser low-pass of the stitch quality (src/dsp_plugin.c:2571-2604). */
+void dsdhip_make_qfir(double *q);
+/* Device scratch of the FIR chain for n input samples per channel, so later
+ * dsdhip_fir_process_dev calls of at most n samples allocate nothing. */
+int dsdhip_fir_reserve(dsdhip_fir_t *f, size_t n);
 
 #ifdef __cplusplus
 }

@@ -209,12 +209,13 @@ __device__ int literal_prune(TrLds<P, HW> &s, int nb, int m, int K,
 
 } // namespace
 
+/* One job on one wave; s is the wave's own LDS region.  Nothing here
+ * synchronises beyond the wave, so several waves of a workgroup may run
+ * different jobs (and different numbers of them). */
 template <int P, int HW>
-__global__ __launch_bounds__(64) void tr_run_jobs(tr_params prm, tr_job *jobs, tr_ctx *ctxs)
+__device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr_ctx *ctxs,
+                                            TrLds<P, HW> &s, const int lane)
 {
-    __shared__ TrLds<P, HW> s;
-    const int lane = threadIdx.x;
-    tr_job *jb = &jobs[blockIdx.x];
     const tr_ctx *c0 = &ctxs[jb->ctx_in];
     const tr_ctx *snap = jb->snap >= 0 ? &ctxs[jb->snap] : nullptr;
     const int K = prm.K, L = prm.L;
@@ -527,8 +528,42 @@ __global__ __launch_bounds__(64) void tr_run_jobs(tr_params prm, tr_job *jobs, t
     }
 }
 
+/* One workgroup (one wave) per job. */
+template <int P, int HW>
+__global__ __launch_bounds__(64) void tr_run_jobs(tr_params prm, tr_job *jobs, tr_ctx *ctxs)
+{
+    __shared__ TrLds<P, HW> s;
+    tr_job *jb = &jobs[blockIdx.x];
+    if (jb->skip) return;
+    tr_job_body<P, HW>(prm, jb, ctxs, s, threadIdx.x);
+}
+
+/* Persistent form: WPB waves per workgroup, each taking the next job slot
+ * from a device counter until `njobs` slots are handed out; slots marked
+ * `skip` are passed over.  Every wave leaves once the counter is past the
+ * end, so the grid drains.  The counter is reset by the caller. */
+template <int P, int HW, int WPB>
+__global__ __launch_bounds__(64 * WPB) void tr_run_queue(tr_params prm, tr_job *jobs, int njobs,
+                                                         int *counter, tr_ctx *ctxs)
+{
+    __shared__ TrLds<P, HW> s[WPB];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (;;) {
+        int j = 0;
+        if (lane == 0) j = atomicAdd(counter, 1);
+        j = __builtin_amdgcn_readfirstlane(j);
+        if (j >= njobs) break;
+        tr_job *jb = &jobs[j];
+        if (jb->skip) continue;
+        tr_job_body<P, HW>(prm, jb, ctxs, s[w], lane);
+    }
+}
+
 #define TR_INST(P, HW) \
-    template __global__ void tr_run_jobs<P, HW>(tr_params, tr_job *, tr_ctx *);
+    template __global__ void tr_run_jobs<P, HW>(tr_params, tr_job *, tr_ctx *); \
+    template __global__ void tr_run_queue<P, HW, 1>(tr_params, tr_job *, int, int *, tr_ctx *); \
+    template __global__ void tr_run_queue<P, HW, 2>(tr_params, tr_job *, int, int *, tr_ctx *); \
+    template __global__ void tr_run_queue<P, HW, 4>(tr_params, tr_job *, int, int *, tr_ctx *);
 #define TR_INST_P(P) TR_INST(P, 1) TR_INST(P, 2) TR_INST(P, 4) TR_INST(P, 8) TR_INST(P, 16) TR_INST(P, 32)
 TR_INST_P(4)
 TR_INST_P(5)
@@ -553,5 +588,29 @@ extern "C" int tr_launch_jobs(const tr_params *prm, tr_job *d_jobs, int njobs,
         return hipGetLastError() == hipSuccess ? 0 : -1; }
 #define TR_CASE_P(PP) TR_CASE(PP, 1) TR_CASE(PP, 2) TR_CASE(PP, 4) TR_CASE(PP, 8) TR_CASE(PP, 16) TR_CASE(PP, 32)
     TR_CASE_P(4) TR_CASE_P(5) TR_CASE_P(6) TR_CASE_P(7) TR_CASE_P(8)
+    return -1;
+}
+
+/* Persistent launch: nblocks workgroups of 64*wpb threads over njobs slots. */
+extern "C" int tr_launch_queue(const tr_params *prm, tr_job *d_jobs, int njobs, int *d_counter,
+                               tr_ctx *d_ctxs, int nblocks, int wpb, hipStream_t stream)
+{
+    if (njobs <= 0) return 0;
+    const int P = prm->P;
+    const int hw_need = (prm->L + 63) / 64;
+    int HW = 1;
+    while (HW < hw_need) HW *= 2;
+    if (P < 4 || P > 8 || HW > 32 || nblocks < 1) return -1;
+    if (wpb != 1 && wpb != 2 && wpb != 4) return -1;
+    dim3 grid((unsigned)nblocks), block(64 * wpb);
+#define TQ_CASE(PP, HH, WW) \
+    if (P == PP && HW == HH && wpb == WW) { \
+        hipLaunchKernelGGL((tr_run_queue<PP, HH, WW>), grid, block, 0, stream, *prm, d_jobs, njobs, \
+                           d_counter, d_ctxs); \
+        return hipGetLastError() == hipSuccess ? 0 : -1; }
+#define TQ_CASE_W(PP, HH) TQ_CASE(PP, HH, 1) TQ_CASE(PP, HH, 2) TQ_CASE(PP, HH, 4)
+#define TQ_CASE_P(PP) TQ_CASE_W(PP, 1) TQ_CASE_W(PP, 2) TQ_CASE_W(PP, 4) TQ_CASE_W(PP, 8) \
+    TQ_CASE_W(PP, 16) TQ_CASE_W(PP, 32)
+    TQ_CASE_P(4) TQ_CASE_P(5) TQ_CASE_P(6) TQ_CASE_P(7) TQ_CASE_P(8)
     return -1;
 }

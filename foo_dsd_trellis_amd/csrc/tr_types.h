@@ -62,7 +62,7 @@ typedef struct tr_job {
     int32_t snap;          /* context followed by foreign links, -1: none */
     int32_t copy;          /* start from a copy of ctx_in (links become foreign) */
     int32_t seed;          /* overwrite every survivor state with seed_st, cost 0 */
-    int32_t pad;
+    int32_t skip;          /* unused slot of a fixed job table: not run */
     double seed_st[TR_MAXP];
     int64_t kept;          /* out: outputs stored */
     int64_t slow_steps;    /* out: steps decided by the literal prune */

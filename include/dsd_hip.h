@@ -209,7 +209,54 @@ int dsdhip_das_last_info(const dsdhip_das_t *d, int *nseg, int64_t *slow_steps, 
                          int max_pos);
 
 /* ======================================================================
- * 6. Differentiable loop filter and pre-emphasis (training).
+ * 6. Batched DAS: many tracks of uneven length in one persistent launch.
+ *    No counterpart in the reference (its DAS runs one stream per plugin
+ *    instance, src/dsp_plugin.c:1941-2761); a track is nch channels of
+ *    float32 PCM in device memory.  Every chunk step (chunk_frames PCM
+ *    frames of every unfinished track: FIR chain, segment plan with
+ *    reposition, one persistent trellis launch over all segments of all
+ *    channels and tracks, stitch, assembly, copy-back) runs on the device
+ *    and is captured once in a hipGraph that is replayed per chunk.  Per
+ *    track and chunk the result equals the single-stream DAS with
+ *    all-parallel seeding and the same segment length.
+ * ====================================================================== */
+typedef struct {
+    int seg_len;            /* segment length; 0: chosen by the autotuner */
+    int waves_per_group;    /* persistent launch: waves per workgroup (1, 2, 4); 0: tuned */
+    int waves_per_cu;       /* persistent launch: resident waves per CU; 0: tuned */
+    int use_graph;          /* 1: capture one chunk step and replay it (default) */
+    double msps_measured;   /* out: trellis Msamples/s of the tuned launch shape (trial) */
+} gpu_hip_tune_t;
+
+typedef struct gpu_hip_plan gpu_hip_plan_t;
+
+/* Plan for tracks of nch channels, fs_in -> fs_out (2^n ratio) with the
+ * chain gain, trellis (ntf_index, depth, cands, lat, state_limit);
+ * tune = NULL tunes everything. */
+gpu_hip_plan_t *gpu_hip_plan_create(unsigned fs_in, unsigned fs_out, double gain, int ntf_index,
+                                    int depth, int cands, int lat, double state_limit, int nch,
+                                    size_t chunk_frames, int reposition,
+                                    const gpu_hip_tune_t *tune);
+/* Use the caller's HIP stream (not the null stream) instead of the plan's own. */
+int gpu_hip_plan_set_stream(gpu_hip_plan_t *p, void *stream);
+/* Registers the tracks: d_pcm[t] = [nch][pcm_stride[t]] float32 device
+ * memory with frames[t] frames; d_out[t] = [nch][out_stride[t]] u64 words
+ * receiving the packed DSD bits (bit = 1 for +1), out_words[t] words per
+ * channel.  NULL strides mean frames[t] / out_words[t]. */
+int gpu_hip_plan_enqueue(gpu_hip_plan_t *p, int ntracks, const float *const *d_pcm,
+                         const int64_t *pcm_stride, const int64_t *frames, uint64_t *const *d_out,
+                         const int64_t *out_stride, const int64_t *out_words);
+/* Runs up to max_chunks chunk steps (< 0: all that remain); the first call
+ * tunes, allocates and captures.  out_bits[t] receives each track's output
+ * bit count so far.  Returns the chunks run, or -1. */
+int gpu_hip_plan_launch_graph(gpu_hip_plan_t *p, int max_chunks, int64_t *out_bits);
+int gpu_hip_plan_get_tune(const gpu_hip_plan_t *p, gpu_hip_tune_t *out);
+int gpu_hip_plan_info(const gpu_hip_plan_t *p, int64_t *chunks_total, int64_t *chunks_done,
+                      int *segments_per_chunk, double *elapsed_ms);
+void gpu_hip_plan_destroy(gpu_hip_plan_t *p);
+
+/* ======================================================================
+ * 7. Differentiable loop filter and pre-emphasis (training).
  *    Replaces the per-sample torch loop of training_pipeline/diff_sdm.py:
  *    61-100 (DiffSDM.forward) and the conv1d of model_pre_sdm.py:44-53
  *    (FixedPreEmphasis.forward).  All pointers are device memory owned by

@@ -171,3 +171,45 @@ size_t ora_fir_chain_ld(unsigned fs_in, unsigned fs_out, const float *in, size_t
 }
 
 void ora_halfband(double *h, int ntaps) { or_design_halfband(h, ntaps, 12.0); }
+
+/* DAS over a stream of chunks with persistent contexts carried between
+ * calls (the plugin's per-chunk DAS, src/dsp_plugin.c:1721-2761). */
+typedef struct {
+    or_ctx *pc;
+    int nch, lat;
+} ora_das_stream_t;
+
+void *ora_das_stream_new(int filter_index, int depth, int cands, int lat, int nch)
+{
+    ora_das_stream_t *h = (ora_das_stream_t *)calloc(1, sizeof(*h));
+    h->pc = (or_ctx *)calloc((size_t)nch, sizeof(or_ctx));
+    h->nch = nch;
+    h->lat = lat;
+    for (int ch = 0; ch < nch; ch++)
+        if (or_sdm_init(&h->pc[ch], or_ntf_get(filter_index), depth, cands, lat) != 0) {
+            free(h->pc);
+            free(h);
+            return NULL;
+        }
+    return h;
+}
+
+long ora_das_stream_chunk(void *hv, const double *in, size_t F, size_t seg_len, int seg0_first,
+                          int reposition, float *out)
+{
+    ora_das_stream_t *h = (ora_das_stream_t *)hv;
+    or_das_plan_t p;
+    or_das_plan_init(&p, F, seg_len, 0, (size_t)64 * (size_t)h->lat, seg0_first, reposition);
+    if (reposition) or_das_reposition(&p, in, F);
+    size_t n = or_das_run(&p, h->pc, h->nch, in, F, out, NULL);
+    or_das_plan_free(&p);
+    return (long)n;
+}
+
+void ora_das_stream_free(void *hv)
+{
+    ora_das_stream_t *h = (ora_das_stream_t *)hv;
+    if (!h) return;
+    free(h->pc);
+    free(h);
+}

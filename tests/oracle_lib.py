@@ -266,3 +266,32 @@ def inband_weights(N, fs, f_hi=20000.0):
     if N % 2 == 0 and w[-1]:
         w[-1] = 1.0
     return w
+
+
+class DasStream:
+    """Oracle DAS over successive chunks with persistent contexts."""
+
+    def __init__(self, lib_, filter_index, depth, cands, lat, nch):
+        L = lib_.lib
+        L.ora_das_stream_new.restype = C.c_void_p
+        L.ora_das_stream_new.argtypes = [C.c_int] * 5
+        L.ora_das_stream_chunk.restype = C.c_long
+        L.ora_das_stream_chunk.argtypes = [C.c_void_p, _dp, C.c_size_t, C.c_size_t, C.c_int, C.c_int, _fp]
+        L.ora_das_stream_free.argtypes = [C.c_void_p]
+        self.L = L
+        self.nch = nch
+        self.h = L.ora_das_stream_new(filter_index, depth, cands, lat, nch)
+
+    def chunk(self, x, seg_len, seg0_first=0, reposition=1):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        F = x.shape[1]
+        out = np.zeros((self.nch, F), np.float32)
+        n = self.L.ora_das_stream_chunk(self.h, _ptr(x, _dp), F, seg_len, seg0_first, reposition,
+                                        _ptr(out, _fp))
+        return out[:, :n]
+
+    def __del__(self):
+        try:
+            self.L.ora_das_stream_free(self.h)
+        except Exception:
+            pass

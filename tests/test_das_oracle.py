@@ -54,3 +54,17 @@ def test_stitch_prefers_converged_region(oracle):
     cur[2000:2600] = prev[2000:2600]
     pos, q, nc = ol.stitch(oracle, prev, cur, 64)
     assert 2000 <= pos < 2600
+
+
+def test_chunk_stream_first_chunk_equals_das(oracle):
+    """The per-chunk oracle DAS on its first chunk is the one-shot DAS, and
+    later chunks continue from the persistent contexts (full output)."""
+    x = _x(2, 3 * 20000, 7)
+    fi = oracle.ntf_auto(2822400)
+    st = ol.DasStream(oracle, fi, 8, 8, 16, 2)
+    a = st.chunk(x[:, :20000], 4096, 0, 1)
+    ref, n, _, _ = ol.das(oracle, fi, 8, 8, 16, x[:, :20000], seg_len=4096, nseg=0, seg0_first=0)
+    assert a.shape[1] == n and np.array_equal(a, ref[:, :n])
+    b = st.chunk(x[:, 20000:40000], 4096, 0, 1)
+    assert b.shape[1] > 0
+    assert set(np.unique(b)) <= {-1.0, 1.0}
