@@ -43,6 +43,19 @@ def flops_per_sample(P, K_):
     return K_ * (5 * P + 6)
 
 
+def traffic_per_input_sample():
+    """HBM bytes per trellis input sample measured with rocprofv3 FETCH_SIZE
+    and WRITE_SIZE (scaled by a copy-kernel calibration), recorded in
+    profiles/tr_run_jobs_dsd256_das_hbm_traffic.md."""
+    import re
+    path = os.path.join(ROOT, "profiles", "tr_run_jobs_dsd256_das_hbm_traffic.md")
+    try:
+        m = re.search(r"traffic_bytes_per_input_sample:\s*([0-9.]+)", open(path).read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
 def cpu_baseline(x_seg, fi, threads, target_s):
     """Oracle trellis segments (K=32, L=64) on `threads` host threads."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -78,6 +91,65 @@ def cpu_baseline(x_seg, fi, threads, target_s):
                       % (threads, per_thread, x_seg.size, keep, wall)}
 
 
+def diff_loop_filter(steps, check_rows):
+    """Differentiable loop filter (64 x 262144 at DSD128, 63-tap
+    pre-emphasis, CLANS-6 CRFB, tanh(v/0.1), in-band rFFT loss): HIP
+    forward + adjoint throughput, and the gradient error against the FP64
+    oracle on `check_rows` rows at temperature 1 (where the relaxed loop
+    contracts; at temperature 10 it is chaotic and long-horizon gradients
+    overflow)."""
+    import torch
+    from foo_dsd_trellis_amd import loopfilter as LF, signals
+    B, N, fs = 64, 262144, 5644800
+    p = torch.from_numpy(signals.multitone_noise(B, N, fs=fs, seed=9)).cuda()
+    pre = LF.FixedPreEmphasis(63)
+    sdm = LF.DiffSDM(rate=fs, temperature=10.0)
+    loss_fn = LF.AudioBandLoss(fs)
+    for _ in range(2):
+        LF.train_step(pre, sdm, loss_fn, p)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        LF.train_step(pre, sdm, loss_fn, p)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    out = {"value": round(B * N / (ms / 1e3) / 1e6, 3), "unit": "Msamples/s (fwd+bwd)",
+           "ms_per_step": round(ms, 3), "temperature": 10.0,
+           "grad_finite": bool(torch.isfinite(sdm.a.grad).all()),
+           "config": "64 x 262144 f32 at DSD128, 63-tap pre-emphasis, %s CRFB, tanh(v/0.1), "
+                     "checkpoint 4096, in-band (0-20 kHz) error power by rFFT" % sdm.ntf_name}
+    if check_rows > 0:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as ol
+        import numpy as np
+        O = ol.oracle()
+        x = p[:check_rows].contiguous()
+        pre1, sdm1 = LF.FixedPreEmphasis(63), LF.DiffSDM(rate=fs, temperature=1.0)
+        LF.train_step(pre1, sdm1, LF.AudioBandLoss(fs), x)
+        xn = x.cpu().numpy()
+        w = pre1.taps.detach().cpu().numpy()
+        a = sdm1.a.detach().cpu().numpy()
+        g = sdm1.g.detach().cpu().numpy()
+        u = ol.pe_forward(O, w, xn)
+        y = ol.lf_forward(O, a, g, 1.0, u)
+        E = np.fft.rfft(y - xn.astype(np.float64), axis=1)
+        wk = ol.inband_weights(N, fs)
+        c = np.full(N // 2 + 1, 2.0)
+        c[0] = c[-1] = 1.0
+        gy = 2.0 * np.fft.irfft(wk * E / c, N, axis=1) / (N * check_rows)
+        gu, ga, gg = ol.lf_backward(O, a, g, 1.0, u, gy)
+        gw = ol.pe_backward(O, 63, xn, gu)
+        rel = lambda v, r: float(np.max(np.abs(v - r)) / max(np.max(np.abs(r)), 1e-300))
+        out["grad_rel_err_vs_fp64"] = max(rel(sdm1.a.grad.cpu().numpy(), ga),
+                                          rel(sdm1.g.grad.cpu().numpy(), gg),
+                                          rel(pre1.taps.grad.cpu().numpy(), gw))
+        out["grad_check"] = "%d rows x %d, temperature 1, max-norm relative error of dL/da, " \
+                            "dL/dg, dL/dtaps vs the FP64 oracle adjoint" % (check_rows, N)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +162,9 @@ def main():
                     help="audio seconds per channel for the plain per-channel GPU form (0: skip)")
     ap.add_argument("--seg-len", type=int, default=SEG_LEN)
     ap.add_argument("--seg0-first", action="store_true")
+    ap.add_argument("--lf-steps", type=int, default=5,
+                    help="differentiable loop filter fwd+bwd steps (0: skip)")
+    ap.add_argument("--lf-check-rows", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
@@ -138,9 +213,11 @@ def main():
     nseg, slow, _ = D.last_info(0)
 
     fps = flops_per_sample(P, K)
+    tb = traffic_per_input_sample()
     achieved = (k_samples * fps) / (k_ms / 1000.0) / 1e12 if k_ms > 0 else 0.0
     roof = {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 6), "traffic": None,
+            "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 6),
+            "traffic": int(tb * k_samples / max(1, k_launches)) if tb else None,
             "kernel": "tr_run_jobs<8,1>", "avg_launch_ms": round(k_ms / max(1, k_launches), 3),
             "flops_per_input_sample": fps, "inputs_per_launch": k_samples // max(1, k_launches)}
 
@@ -167,6 +244,8 @@ def main():
         seg = d_f[0, SEG_LEN:SEG_LEN + SEG_LEN + 2 * 64 * L].cpu().numpy()
         cpu = cpu_baseline(np.ascontiguousarray(seg), fi, thr, args.cpu_seconds)
 
+    lf = diff_loop_filter(args.lf_steps, args.lf_check_rows) if args.lf_steps > 0 else None
+
     line = {
         "metric": "Trellis SDM Msamples/s & xrealtime at DSD256 K=32, batched vs plain",
         "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
@@ -182,6 +261,7 @@ def main():
         "roofline": roof,
         "plain_gpu": plain,
         "cpu_baseline": cpu,
+        "diff_loop_filter": lf,
     }
     print(json.dumps(line), flush=True)
 
