@@ -82,26 +82,42 @@ __device__ __forceinline__ u64 lt_sweep(const double *key, double c)
     return ((u64)hi << 32) | lo;
 }
 
-/* LDS working set of one context. */
+/* LDS working set of one context: the per-slot records of both banks
+ * (followed by the path-hash chains), the prune's tables, and the lane
+ * holding each survivor.  The survivors' own records (loop-filter state,
+ * metric, path, traceback bit, history) stay in the registers of the lane
+ * that produced them and are fetched by the next step's children with
+ * ds_bpermute, which keeps LDS per wave small (more waves per CU). */
 template <int P, int HW>
 struct TrLds {
-    double st[2][P][TR_MAXK];      /* [bank][q][k]: lanes of a step read consecutive k */
-    double cost[2][TR_MAXK];
-    u64 hist[2][TR_MAXK][HW];
-    uint32_t path[2][TR_MAXK];
-    uint8_t next[2][TR_MAXK];
-    double key[64];
+    double ncost[2][64];
+    uint32_t npath[2][64];
     u64 gtab[256 + 64];            /* + one scratch entry per lane */
     u64 btab[128 + 64];
-    uint32_t npath[2][64];
-    double ncost[2][64];
     int8_t link[2][64];
     uint8_t rank[64];
     uint8_t nnext[64];
     int8_t head[128];
     int8_t act[TR_MAXK + 1];
+    int8_t src[TR_MAXK];           /* lane holding the survivor of each rank */
     int32_t scal[4];
 };
+
+/* value of another lane's register (all lanes active) */
+__device__ __forceinline__ int bp_i(int v, int from) { return __builtin_amdgcn_ds_bpermute(from << 2, v); }
+__device__ __forceinline__ double bp_d(double v, int from)
+{
+    const u64 b = (u64)__double_as_longlong(v);
+    const unsigned lo = (unsigned)bp_i((int)(unsigned)b, from);
+    const unsigned hi = (unsigned)bp_i((int)(unsigned)(b >> 32), from);
+    return __longlong_as_double((long long)(((u64)hi << 32) | lo));
+}
+__device__ __forceinline__ u64 bp_u64(u64 v, int from)
+{
+    const unsigned lo = (unsigned)bp_i((int)(unsigned)v, from);
+    const unsigned hi = (unsigned)bp_i((int)(unsigned)(v >> 32), from);
+    return ((u64)hi << 32) | lo;
+}
 
 /* Literal sequential prune (src/trellis.c:582-731), run by one lane over the
  * children held in LDS.  Writes rank[] (255 = pruned) and returns the number
@@ -226,19 +242,28 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
     /* ---- load the context ---- */
     int num_cands = c0->num_cands, pos = c0->pos, pending = c0->pending, idx = c0->idx;
     const int draining = c0->draining;
+    /* survivor record held by this lane, and its rank (255: none) */
+    double rst[P], rcost = 0.0;
+    uint32_t rpath = 0;
+    int rnext = 0, myr = 255;
+    u64 rhist[HW];
+#pragma unroll
+    for (int q = 0; q < P; q++) rst[q] = 0.0;
+#pragma unroll
+    for (int w = 0; w < HW; w++) rhist[w] = 0;
     {
-        const int cb = idx;
-        for (int i = lane; i < num_cands * P; i += 64) {
-            const int k = i / P, q = i % P;
-            s.st[cb][q][k] = jb->seed ? jb->seed_st[q] : c0->st[k][q];
-        }
-        for (int i = lane; i < num_cands * HW; i += 64)
-            s.hist[cb][i / HW][i % HW] = c0->hist[i / HW][i % HW];
         if (lane < num_cands) {
-            s.cost[cb][lane] = jb->seed ? 0.0 : c0->cost[lane];
-            s.path[cb][lane] = c0->path[lane];
-            s.next[cb][lane] = c0->next[lane];
+            const bool sd = jb->seed != 0;
+#pragma unroll
+            for (int q = 0; q < P; q++) rst[q] = sd ? jb->seed_st[q] : c0->st[lane][q];
+#pragma unroll
+            for (int w = 0; w < HW; w++) rhist[w] = c0->hist[lane][w];
+            rcost = sd ? 0.0 : c0->cost[lane];
+            rpath = c0->path[lane];
+            rnext = c0->next[lane];
+            myr = lane;
         }
+        if (lane < TR_MAXK) s.src[lane] = (int8_t)lane;
         for (int b = 0; b < 2; b++) {
             int8_t l = c0->link[b][lane];
             if (jb->copy) {
@@ -271,7 +296,7 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
         }
         const double x = rl_d(xv, tl);
 
-        const int cb = idx, nb = idx ^ 1;
+        const int nb = idx ^ 1;
         const int n = num_cands, m = 2 * n;
         const bool live = lane < m;
         const int k = lane >> 1, br = lane & 1;
@@ -279,13 +304,21 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
         if (next_pos == L) next_pos = 0;
 
         /* ---- 1. expand ---- */
+        const int from = s.src[k] & 63;        /* lane holding parent k (k >= n: unused) */
         double sp[P], d[P];
 #pragma unroll
-        for (int q = 0; q < P; q++) sp[q] = s.st[cb][q][k];
-        const double pcost = s.cost[cb][k];
-        const uint32_t ppath = s.path[cb][k];
-        const int pnext = s.next[cb][k];
-        const int nnew = (int)((s.hist[cb][k][next_pos >> 6] >> (next_pos & 63)) & 1ull);
+        for (int q = 0; q < P; q++) sp[q] = bp_d(rst[q], from);
+        const double pcost = bp_d(rcost, from);
+        const uint32_t ppath = (uint32_t)bp_i((int)rpath, from);
+        const int pnext = bp_i(rnext, from);
+        u64 phist[HW];
+#pragma unroll
+        for (int w = 0; w < HW; w++) phist[w] = bp_u64(rhist[w], from);
+        u64 hnext = phist[0];
+#pragma unroll
+        for (int w = 1; w < HW; w++)
+            if (w == (next_pos >> 6)) hnext = phist[w];
+        const int nnew = (int)((hnext >> (next_pos & 63)) & 1ull);
 
         d[0] = sp[0] - prm.g[0] * sp[1] + x - 0.0;
         double v = x + prm.a[0] * d[0];
@@ -312,10 +345,9 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
         const int cnext = pnext;
         if (ballot(live && !__builtin_isfinite(cost))) err = 1;
 
-        s.key[lane] = cost;   /* entries >= m are never counted */
         if (live) {
             s.npath[nb][lane] = path;
-            s.ncost[nb][lane] = cost;
+            s.ncost[nb][lane] = cost;      /* also the sweep's keys: entries >= m are never counted */
         }
         wsync();
 
@@ -323,8 +355,9 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
         const u64 ALL = (m == 64) ? ~0ull : ((1ull << m) - 1ull);
         /* LT bit j: child j has a strictly smaller metric (bits >= m are
          * don't-care; every use masks them off) */
-        const u64 LT = (m <= 16) ? lt_sweep<16>(s.key, cost)
-                     : (m <= 32) ? lt_sweep<32>(s.key, cost) : lt_sweep<64>(s.key, cost);
+        const double *keys = s.ncost[nb];
+        const u64 LT = (m <= 16) ? lt_sweep<16>(keys, cost)
+                     : (m <= 32) ? lt_sweep<32>(keys, cost) : lt_sweep<64>(keys, cost);
 
         /* equal metrics: strict ranks must be distinct */
         const int rall = __popcll(LT & ALL);
@@ -452,23 +485,30 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
         const int r0 = ff1(B0);
         const int outbit = (int)((N1 >> r0) & 1);
         const double minc = rl_d(cost, r0);
-        if (surv) {
+        /* every lane takes its child's record; only survivors' records are
+         * referenced afterwards (through src[] and myr), so the previous
+         * record is dead here and shares registers with the new one */
 #pragma unroll
-            for (int q = 0; q < P; q++) s.st[nb][q][myrank] = d[q];
-            const double nc = cost - minc;
-            s.cost[nb][myrank] = nc;
-            s.ncost[nb][lane] = nc;
-            s.path[nb][myrank] = path;
-            s.next[nb][myrank] = (uint8_t)nnew;
+        for (int q = 0; q < P; q++) rst[q] = d[q];
+        const double nc = cost - minc;
+        rcost = nc;
+        rpath = path;
+        rnext = nnew;
+        {
             const int pw = pos >> 6;
             const u64 pbit = 1ull << (pos & 63);
 #pragma unroll
             for (int w = 0; w < HW; w++) {
-                u64 hv = s.hist[cb][k][w];
+                u64 hv = phist[w];
                 if (w == pw) hv = (path & 1u) ? (hv | pbit) : (hv & ~pbit);
-                s.hist[nb][myrank][w] = hv;
+                rhist[w] = hv;
             }
         }
+        if (surv) {
+            s.ncost[nb][lane] = nc;
+            s.src[myrank] = (int8_t)lane;
+        }
+        myr = surv ? myrank : 255;
         wsync();
         num_cands = nsurv;
         pos = next_pos;
@@ -495,13 +535,14 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
     /* ---- store the final context ---- */
     if (jb->ctx_out >= 0) {
         tr_ctx *co = &ctxs[jb->ctx_out];
-        const int cb = idx;
-        for (int i = lane; i < num_cands * P; i += 64) co->st[i / P][i % P] = s.st[cb][i % P][i / P];
-        for (int i = lane; i < num_cands * HW; i += 64) co->hist[i / HW][i % HW] = s.hist[cb][i / HW][i % HW];
-        if (lane < num_cands) {
-            co->cost[lane] = s.cost[cb][lane];
-            co->path[lane] = s.path[cb][lane];
-            co->next[lane] = s.next[cb][lane];
+        if (myr < num_cands) {
+#pragma unroll
+            for (int q = 0; q < P; q++) co->st[myr][q] = rst[q];
+#pragma unroll
+            for (int w = 0; w < HW; w++) co->hist[myr][w] = rhist[w];
+            co->cost[myr] = rcost;
+            co->path[myr] = rpath;
+            co->next[myr] = (uint8_t)rnext;
         }
         bool fr = false;
         for (int b = 0; b < 2; b++) {
@@ -528,9 +569,18 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
     }
 }
 
+/* Occupancy target: with the survivor records in registers a wave needs
+ * 6 KB of LDS, so registers set the waves per SIMD; up to 128-sample
+ * histories the kernel fits 80 VGPRs (6 waves per SIMD, 24 per CU) with a
+ * few spills outside the step loop.  Wider histories keep their registers. */
+template <int HW>
+struct TrWaves {
+    static constexpr int value = HW <= 2 ? 6 : 1;
+};
+
 /* One workgroup (one wave) per job. */
 template <int P, int HW>
-__global__ __launch_bounds__(64) void tr_run_jobs(tr_params prm, tr_job *jobs, tr_ctx *ctxs)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TrWaves<HW>::value, 8))) void tr_run_jobs(tr_params prm, tr_job *jobs, tr_ctx *ctxs)
 {
     __shared__ TrLds<P, HW> s;
     tr_job *jb = &jobs[blockIdx.x];
@@ -543,7 +593,7 @@ __global__ __launch_bounds__(64) void tr_run_jobs(tr_params prm, tr_job *jobs, t
  * `skip` are passed over.  Every wave leaves once the counter is past the
  * end, so the grid drains.  The counter is reset by the caller. */
 template <int P, int HW, int WPB>
-__global__ __launch_bounds__(64 * WPB) void tr_run_queue(tr_params prm, tr_job *jobs, int njobs,
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(TrWaves<HW>::value, 8))) void tr_run_queue(tr_params prm, tr_job *jobs, int njobs,
                                                          int *counter, tr_ctx *ctxs)
 {
     __shared__ TrLds<P, HW> s[WPB];
