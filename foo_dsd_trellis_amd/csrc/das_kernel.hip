@@ -286,3 +286,33 @@ extern "C" __global__ void bits_to_f32(const u64 *__restrict__ bits, int64_t wor
     const u64 w = bits[(int64_t)ch * words_per_ch + (i >> 6)];
     out[(int64_t)ch * out_stride + i] = ((w >> (i & 63)) & 1ull) ? 1.0f : -1.0f;
 }
+
+/* Appends n packed bits of each channel (src row ch, bit 0 on) at bit
+ * position pos of the channel's destination row; bits of dst below pos keep
+ * their value, bits from pos + n to the end of the last written word become
+ * 0.  One lane per destination word touched. */
+extern "C" __global__ void bits_append(const u64 *__restrict__ src, int64_t src_words,
+                                       u64 *__restrict__ dst, int64_t dst_stride,
+                                       int64_t dst_words, int64_t pos, int64_t n)
+{
+    const int ch = blockIdx.y;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t w = pos / 64 + i;
+    if (n <= 0 || w * 64 >= pos + n || w >= dst_words) return;
+    const u64 *s = src + (int64_t)ch * src_words;
+    u64 *d = dst + (int64_t)ch * dst_stride;
+    u64 word = (w * 64 < pos) ? d[w] : 0ull;
+    for (int b = 0; b < 64; b++) {
+        const int64_t g = w * 64 + b;            /* destination bit */
+        if (g < pos) continue;
+        const u64 mask = 1ull << b;
+        if (g >= pos + n) {
+            word &= ~mask;
+            continue;
+        }
+        const int64_t sb = g - pos;              /* source bit, < n <= 64 * src_words */
+        const u64 bit = (s[sb >> 6] >> (sb & 63)) & 1ull;
+        word = (word & ~mask) | (bit << b);
+    }
+    d[w] = word;
+}

@@ -32,6 +32,14 @@ __global__ void bt_finish(bt_cfg, const int *, tr_ctx *);
 __global__ void bt_advance(bt_cfg, bt_track *, const int64_t *, bt_state *);
 __global__ void das_stitch(const u64 *, const int64_t *, const int64_t *, const int *, int,
                            const double *, int *, int *, int *, int *, double *);
+__global__ void bits_append(const u64 *, int64_t, u64 *, int64_t, int64_t, int64_t, int64_t);
+int tr_occupancy(const tr_params *prm);
+dsdhip_das_t *dsdhip_das_create_tracks(int ntf_index, int depth, int cands, int lat,
+                                       double state_limit, int nch, int ntracks, size_t seg_len,
+                                       int nseg, int seg0_first, int reposition);
+int dsdhip_das_run_tracks_dev(dsdhip_das_t *d, const double *const *d_in, const int64_t *in_stride,
+                              const size_t *F, unsigned long long *const *d_out,
+                              const int64_t *out_words, size_t *out_count);
 }
 
 namespace {
@@ -96,10 +104,20 @@ struct gpu_hip_plan {
     hipGraphExec_t exec;
     hipEvent_t ev0, ev1;
     double ms_total;
+    /* host-orchestrated chunk step */
+    int host_ready;
+    dsdhip_das_t *das;
+    u64 *d_tmp;                 /* [ns][tmp_words] packed bits of one chunk */
+    int64_t tmp_words;
+    std::vector<int64_t> out_pos;
 };
 
 static void plan_release(gpu_hip_plan_t *p)
 {
+    dsdhip_das_destroy(p->das);
+    p->das = nullptr;
+    dfree(p->d_tmp);
+    p->host_ready = 0;
     if (p->exec) hipGraphExecDestroy(p->exec);
     if (p->graph) hipGraphDestroy(p->graph);
     p->exec = nullptr;
@@ -366,6 +384,170 @@ static int plan_setup(gpu_hip_plan_t *p)
     return 0;
 }
 
+/* ---- host-orchestrated chunk step ----------------------------------------
+ * Every track's chunk goes through the FIR chain together (one chain over all
+ * nch * ntracks streams) and through one multi-track DAS call: all segments of
+ * all tracks in one trellis launch, all stitch boundaries in one launch.
+ * Host work per chunk: plans and reposition, stitch selection, span tables.
+ * The tuner picks the segment length from the measured throughput of the
+ * trellis launch (trial jobs at full residency) and the chunk's makespan
+ * model; the launch shape of this path is one wave per workgroup with the
+ * kernel's own residency (reported in waves_per_group / waves_per_cu). */
+static int host_tune(gpu_hip_plan_t *p)
+{
+    const int cus = cu_count();
+    const int64_t O = 64LL * p->L;
+    dsdhip_trellis_t *probe = dsdhip_trellis_create(p->ntf, p->depth, p->K, p->L, p->state_limit, 1);
+    if (!probe) return -1;
+    tr_params prm;
+    dsdhip_trellis_params(probe, &prm);
+    dsdhip_trellis_destroy(probe);
+    const int occ = tr_occupancy(&prm);
+    if (occ <= 0) return -1;
+    const int slots = cus * occ;
+    p->tune.waves_per_group = 1;
+    p->tune.waves_per_cu = occ;
+    if (p->tune.seg_len > 0) return 0;
+    /* trial: two full rounds of jobs from a fresh context */
+    const int trial_n = 8192, nj = 2 * slots;
+    dsdhip_trellis_t *tt = dsdhip_trellis_create(p->ntf, p->depth, p->K, p->L, p->state_limit, 1 + nj);
+    if (!tt) return -1;
+    dsdhip_trellis_set_stream(tt, p->stream);
+    std::vector<double> x((size_t)trial_n);
+    uint64_t r = 0x9e3779b97f4a7c15ull;
+    for (int i = 0; i < trial_n; i++) {
+        r = r * 6364136223846793005ull + 1442695040888963407ull;
+        const double u = (double)(r >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+        x[(size_t)i] = 0.12 * sin(2.0 * 3.14159265358979 * 1000.0 * i / (double)p->fs_out) + 0.2 * u;
+    }
+    double *d_x = nullptr;
+    u64 *d_o = nullptr;
+    int rc = -1;
+    if (dalloc(&d_x, (size_t)trial_n) == 0 && dalloc(&d_o, (size_t)nj * (trial_n / 64 + 2)) == 0 &&
+        hipMemcpy(d_x, x.data(), sizeof(double) * x.size(), hipMemcpyHostToDevice) == hipSuccess) {
+        std::vector<int> ctx((size_t)nj), src((size_t)nj, 0);
+        std::vector<const double *> in((size_t)nj, d_x);
+        std::vector<int64_t> n((size_t)nj, trial_n), keep((size_t)nj, 0), kept((size_t)nj), slow((size_t)nj);
+        std::vector<uint64_t *> out((size_t)nj);
+        for (int j = 0; j < nj; j++) {
+            ctx[(size_t)j] = 1 + j;
+            out[(size_t)j] = (uint64_t *)(d_o + (size_t)j * (trial_n / 64 + 2));
+        }
+        hipEvent_t e0, e1;
+        hipEventCreate(&e0);
+        hipEventCreate(&e1);
+        hipEventRecord(e0, p->stream);
+        int err = 0;
+        const int jr = dsdhip_trellis_run_jobs(tt, nj, ctx.data(), src.data(), in.data(), n.data(),
+                                               keep.data(), out.data(), nullptr, kept.data(),
+                                               slow.data(), &err);
+        hipEventRecord(e1, p->stream);
+        hipEventSynchronize(e1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        if (jr == 0 && ms > 0.f) {
+            const double rate = (double)nj * trial_n / (ms * 1e-3);     /* samples/s at full residency */
+            p->tune.msps_measured = rate / 1e6;
+            const int64_t Ffull = p->CF * p->R;
+            double best_cost = 1e300;
+            int64_t best_len = 65536;
+            for (int64_t len = 16384; len <= 262144; len *= 2) {
+                if (len < 4 * O || len > Ffull) continue;
+                int64_t jobs = 0;
+                for (int t = 0; t < p->ntracks; t++) {
+                    const int64_t v = std::min(p->CF, p->tracks[(size_t)t].frames);
+                    if (v > 0) jobs += (int64_t)p->nch * segs_for(v * p->R, len);
+                }
+                const int64_t rounds = (jobs + slots - 1) / slots;
+                const double makespan = (double)rounds * (double)slots * (double)(len + 2 * O) / rate;
+                const double cost = makespan / ((double)jobs * (double)len);
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best_len = len;
+                }
+            }
+            p->tune.seg_len = (int)best_len;
+            rc = 0;
+        }
+    }
+    dfree(d_x);
+    dfree(d_o);
+    dsdhip_trellis_destroy(tt);
+    return rc;
+}
+
+static int host_setup(gpu_hip_plan_t *p)
+{
+    if (!p->tuned) {
+        if (host_tune(p) != 0) return -1;
+        p->tuned = 1;
+    }
+    const int ns = p->ntracks * p->nch;
+    const int64_t Ffull = p->CF * p->R;
+    p->fir = dsdhip_fir_create(p->fs_in, p->fs_out, p->gain, ns);
+    if (!p->fir) return -1;
+    dsdhip_fir_set_stream(p->fir, p->stream);
+    p->das = dsdhip_das_create_tracks(p->ntf, p->depth, p->K, p->L, p->state_limit, p->nch,
+                                      p->ntracks, (size_t)p->tune.seg_len, 0, 0, p->reposition);
+    if (!p->das) return -1;
+    dsdhip_das_set_stream(p->das, p->stream);
+    p->tmp_words = Ffull / 64 + 2;
+    if (dalloc(&p->d_stage, (size_t)ns * (size_t)p->CF) || dalloc(&p->d_f, (size_t)ns * (size_t)Ffull) ||
+        dalloc(&p->d_tmp, (size_t)ns * (size_t)p->tmp_words))
+        return -1;
+    p->out_pos.assign((size_t)p->ntracks, 0);
+    p->host_ready = 1;
+    return 0;
+}
+
+static int host_step(gpu_hip_plan_t *p)
+{
+    const int nch = p->nch, T = p->ntracks, ns = T * nch;
+    const int64_t CF = p->CF, Ffull = CF * p->R;
+    hipStream_t st = p->stream;
+    if (hipMemsetAsync(p->d_stage, 0, sizeof(float) * (size_t)ns * (size_t)CF, st) != hipSuccess) return -1;
+    std::vector<int64_t> v((size_t)T);
+    for (int t = 0; t < T; t++) {
+        const bt_track &k = p->tracks[(size_t)t];
+        int64_t vv = k.frames - p->chunk * CF;
+        vv = vv < 0 ? 0 : (vv > CF ? CF : vv);
+        v[(size_t)t] = vv;
+        if (vv > 0 &&
+            hipMemcpy2DAsync(p->d_stage + (size_t)t * nch * CF, sizeof(float) * (size_t)CF,
+                             k.pcm + p->chunk * CF, sizeof(float) * (size_t)k.pcm_stride,
+                             sizeof(float) * (size_t)vv, (size_t)nch, hipMemcpyDeviceToDevice, st) != hipSuccess)
+            return -1;
+    }
+    if (dsdhip_fir_process_dev(p->fir, p->d_stage, CF, (size_t)CF, p->d_f, Ffull) != 0) return -1;
+    std::vector<const double *> din((size_t)T);
+    std::vector<int64_t> stride((size_t)T, Ffull), words((size_t)T, p->tmp_words);
+    std::vector<size_t> F((size_t)T), cnt((size_t)T, 0);
+    std::vector<unsigned long long *> dout((size_t)T);
+    for (int t = 0; t < T; t++) {
+        din[(size_t)t] = p->d_f + (size_t)t * nch * Ffull;
+        F[(size_t)t] = (size_t)(v[(size_t)t] * p->R);
+        dout[(size_t)t] = (unsigned long long *)(p->d_tmp + (size_t)t * nch * p->tmp_words);
+    }
+    if (dsdhip_das_run_tracks_dev(p->das, din.data(), stride.data(), F.data(), dout.data(),
+                                  words.data(), cnt.data()) != 0)
+        return -1;
+    for (int t = 0; t < T; t++) {
+        const int64_t n = (int64_t)cnt[(size_t)t];
+        if (n <= 0) continue;
+        const bt_track &k = p->tracks[(size_t)t];
+        const int64_t pos = p->out_pos[(size_t)t];
+        if ((pos + n + 63) / 64 > k.out_words) return -1;
+        const int64_t touched = (pos + n + 63) / 64 - pos / 64;
+        hipLaunchKernelGGL(bits_append, dim3((unsigned)((touched + 255) / 256), (unsigned)nch), dim3(256), 0,
+                           st, (const u64 *)dout[(size_t)t], p->tmp_words, k.out, k.out_stride,
+                           k.out_words, pos, n);
+        p->out_pos[(size_t)t] = pos + n;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 extern "C" {
 
 gpu_hip_plan_t *gpu_hip_plan_create(unsigned fs_in, unsigned fs_out, double gain, int ntf_index,
@@ -455,15 +637,37 @@ int gpu_hip_plan_enqueue(gpu_hip_plan_t *p, int ntracks, const float *const *d_p
     return 0;
 }
 
-/* The first run of a chunk step on the GPU ended in a memory fault whose
- * cause has not been found (the batch kernels' indexing checks out in a
- * CPU emulation of the pipeline); until it is, launching is refused. */
+/* The device-side chunk step (use_graph = 1) ended in a GPU memory fault on
+ * its first run, with the cause not yet found (the batch kernels' indexing
+ * checks out in a CPU emulation of the pipeline); until it is, plans run the
+ * host-orchestrated batched step instead. */
 static const int kPlanLaunchEnabled = 0;
 
 int gpu_hip_plan_launch_graph(gpu_hip_plan_t *p, int max_chunks, int64_t *out_bits)
 {
     if (!p || p->ntracks < 1) return -1;
-    if (!kPlanLaunchEnabled) return -1;
+    if (!p->tune.use_graph || !kPlanLaunchEnabled) {
+        /* host-orchestrated batched chunk step */
+        p->tune.use_graph = 0;
+        if (!p->host_ready && host_setup(p) != 0) {
+            plan_release(p);
+            return -1;
+        }
+        int64_t nh = p->nchunks - p->chunk;
+        if (max_chunks >= 0 && nh > max_chunks) nh = max_chunks;
+        hipEventRecord(p->ev0, p->stream);
+        for (int64_t i = 0; i < nh; i++) {
+            if (host_step(p) != 0) return -1;
+            p->chunk++;
+        }
+        hipEventRecord(p->ev1, p->stream);
+        if (hipStreamSynchronize(p->stream) != hipSuccess) return -1;
+        float hms = 0.f;
+        if (hipEventElapsedTime(&hms, p->ev0, p->ev1) == hipSuccess) p->ms_total += hms;
+        if (out_bits)
+            for (int t = 0; t < p->ntracks; t++) out_bits[t] = p->out_pos[(size_t)t];
+        return (int)nh;
+    }
     if (!p->ready && plan_setup(p) != 0) {
         plan_release(p);
         return -1;
@@ -509,7 +713,9 @@ int gpu_hip_plan_info(const gpu_hip_plan_t *p, int64_t *chunks_total, int64_t *c
     if (!p) return -1;
     if (chunks_total) *chunks_total = p->nchunks;
     if (chunks_done) *chunks_done = p->chunk;
-    if (segments_per_chunk) *segments_per_chunk = p->ready ? p->cfg.smax : 0;
+    if (segments_per_chunk)
+        *segments_per_chunk = p->ready ? p->cfg.smax
+                              : (p->host_ready ? segs_for(p->CF * p->R, p->tune.seg_len) : 0);
     if (elapsed_ms) *elapsed_ms = p->ms_total;
     return 0;
 }

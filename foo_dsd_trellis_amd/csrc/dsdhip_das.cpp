@@ -46,6 +46,7 @@ static double nan_val() { return __builtin_nan(""); }
 struct dsdhip_das {
     double a[10], g[10];
     int order, depth, K, L, nch;
+    int ntracks, ns;             /* tracks of nch channels; ns = ntracks * nch streams */
     double state_limit;
     size_t O;
     size_t seg_len;
@@ -93,10 +94,10 @@ extern "C" void dsdhip_make_qfir(double *q)
     for (int i = 0; i < N; i++) q[i] /= sum;
 }
 
-/* context indices in the pool */
-static inline int ix_persist(const dsdhip_das_t *, int ch) { return ch; }
-static inline int ix_snap(const dsdhip_das_t *d, int ch) { return d->nch + ch; }
-static inline int ix_seg(const dsdhip_das_t *d, int ch, int seg) { return 2 * d->nch + ch * d->cap_seg + seg; }
+/* context indices in the pool; stream s = track * nch + channel */
+static inline int ix_persist(const dsdhip_das_t *, int s) { return s; }
+static inline int ix_snap(const dsdhip_das_t *d, int s) { return d->ns + s; }
+static inline int ix_seg(const dsdhip_das_t *d, int s, int seg) { return 2 * d->ns + s * d->cap_seg + seg; }
 
 template <typename T>
 struct DevBuf {
@@ -117,12 +118,12 @@ struct DevBuf {
 
 extern "C" {
 
-dsdhip_das_t *dsdhip_das_create_coeffs(const double *a, const double *g, int order, int depth,
-                                       int cands, int lat, double state_limit, int nch,
-                                       size_t seg_len, int nseg, int seg0_first, int reposition)
+static dsdhip_das_t *das_create(const double *a, const double *g, int order, int depth, int cands,
+                                int lat, double state_limit, int nch, int ntracks, size_t seg_len,
+                                int nseg, int seg0_first, int reposition)
 {
     if (!a || !g || order < 4 || order > TR_MAXP) return NULL;
-    if (nch < 1 || lat < 1 || (seg_len == 0 && nseg < 1)) return NULL;
+    if (nch < 1 || ntracks < 1 || lat < 1 || (seg_len == 0 && nseg < 1)) return NULL;
     if ((size_t)64 * (size_t)lat > 8192) return NULL;   /* overlap held in LDS by the stitch kernel */
     if (seg_len > 0 && seg_len < (size_t)128 * (size_t)lat) return NULL;   /* segments >= 2 overlaps */
     dsdhip_das_t *d = new dsdhip_das();
@@ -135,6 +136,8 @@ dsdhip_das_t *dsdhip_das_create_coeffs(const double *a, const double *g, int ord
     d->K = cands;
     d->L = lat;
     d->nch = nch;
+    d->ntracks = ntracks;
+    d->ns = nch * ntracks;
     d->state_limit = state_limit;
     d->O = (size_t)64 * (size_t)lat;
     d->seg_len = seg_len;
@@ -156,7 +159,7 @@ dsdhip_das_t *dsdhip_das_create_coeffs(const double *a, const double *g, int ord
         return NULL;
     }
     /* persistent contexts exist from the start (fresh) */
-    d->tr = dsdhip_trellis_create_coeffs(d->a, d->g, order, depth, cands, lat, state_limit, 2 * nch);
+    d->tr = dsdhip_trellis_create_coeffs(d->a, d->g, order, depth, cands, lat, state_limit, 2 * d->ns);
     if (!d->tr) {
         hipFree(d->d_qfir);
         delete d;
@@ -165,13 +168,31 @@ dsdhip_das_t *dsdhip_das_create_coeffs(const double *a, const double *g, int ord
     return d;
 }
 
+dsdhip_das_t *dsdhip_das_create_coeffs(const double *a, const double *g, int order, int depth,
+                                       int cands, int lat, double state_limit, int nch,
+                                       size_t seg_len, int nseg, int seg0_first, int reposition)
+{
+    return das_create(a, g, order, depth, cands, lat, state_limit, nch, 1, seg_len, nseg,
+                      seg0_first, reposition);
+}
+
 dsdhip_das_t *dsdhip_das_create(int ntf_index, int depth, int cands, int lat, double state_limit,
                                 int nch, size_t seg_len, int nseg, int seg0_first, int reposition)
 {
     const dsdhip_ntf_t *f = dsdhip_ntf_get(ntf_index);
     if (!f) return NULL;
-    return dsdhip_das_create_coeffs(f->a, f->g, f->order, depth, cands, lat, state_limit, nch,
-                                    seg_len, nseg, seg0_first, reposition);
+    return das_create(f->a, f->g, f->order, depth, cands, lat, state_limit, nch, 1, seg_len, nseg,
+                      seg0_first, reposition);
+}
+
+dsdhip_das_t *dsdhip_das_create_tracks(int ntf_index, int depth, int cands, int lat,
+                                       double state_limit, int nch, int ntracks, size_t seg_len,
+                                       int nseg, int seg0_first, int reposition)
+{
+    const dsdhip_ntf_t *f = dsdhip_ntf_get(ntf_index);
+    if (!f) return NULL;
+    return das_create(f->a, f->g, f->order, depth, cands, lat, state_limit, nch, ntracks, seg_len,
+                      nseg, seg0_first, reposition);
 }
 
 void dsdhip_das_destroy(dsdhip_das_t *d)
@@ -231,13 +252,13 @@ static int ensure_pool(dsdhip_das_t *d, int S)
     if (S <= d->cap_seg) return 0;
     int cap = S;
     dsdhip_trellis_t *nt = dsdhip_trellis_create_coeffs(d->a, d->g, d->order, d->depth, d->K, d->L,
-                                                        d->state_limit, 2 * d->nch + d->nch * cap);
+                                                        d->state_limit, 2 * d->ns + d->ns * cap);
     if (!nt) return -1;
     dsdhip_trellis_set_stream(nt, d->stream);
-    for (int ch = 0; ch < d->nch; ch++) {
+    for (int s = 0; s < d->ns; s++) {
         tr_ctx c;
-        if (dsdhip_trellis_get(d->tr, ix_persist(d, ch), &c) != 0) return -1;
-        if (dsdhip_trellis_put(nt, ch, &c) != 0) return -1;
+        if (dsdhip_trellis_get(d->tr, ix_persist(d, s), &c) != 0) return -1;
+        if (dsdhip_trellis_put(nt, s, &c) != 0) return -1;
     }
     if (d->timing) {
         double ms;
@@ -255,8 +276,8 @@ static int ensure_pool(dsdhip_das_t *d, int S)
 int dsdhip_das_reset(dsdhip_das_t *d)
 {
     if (!d) return -1;
-    for (int ch = 0; ch < d->nch; ch++)
-        if (dsdhip_trellis_reset(d->tr, ix_persist(d, ch)) != 0) return -1;
+    for (int s = 0; s < d->ns; s++)
+        if (dsdhip_trellis_reset(d->tr, ix_persist(d, s)) != 0) return -1;
     return 0;
 }
 
@@ -368,110 +389,161 @@ static int reposition(dsdhip_das_t *d, const double *d_ch0, size_t F, std::vecto
  * in_stride); d_out: nch rows of out_words packed-bit words (1 = +1.0).
  * *out_count receives the assembled length (channel 0's, as the reference).
  */
-int dsdhip_das_run_dev(dsdhip_das_t *d, const double *d_in, int64_t in_stride, size_t F,
-                       unsigned long long *d_out, int64_t out_words, size_t *out_count)
+/* One track of a run: its plan, segment geometry and results. */
+struct TrackRun {
+    const double *in = nullptr;
+    int64_t in_stride = 0;
+    size_t F = 0;
+    unsigned long long *out = nullptr;
+    int64_t out_words = 0;
+    int S = 0;
+    std::vector<size_t> st, sz;
+    std::vector<int64_t> in_start, in_cnt, keep;   /* per segment */
+    std::vector<int64_t> woff;                     /* per (ch, seg): word offset in segbits */
+    std::vector<int64_t> kept, slow;               /* per (ch, seg) */
+    std::vector<int> bp;                           /* stitch position per segment */
+    int64_t total = 0;
+};
+
+/* DAS over every track (stream s = t * nch + ch) with all segments of all
+ * tracks in one trellis launch (two with seg0-first seeding), all
+ * boundaries in one stitch launch.  Tracks with F[t] == 0 are left alone. */
+static int run_tracks(dsdhip_das_t *d, const double *const *d_in, const int64_t *in_stride,
+                      const size_t *F, unsigned long long *const *d_out, const int64_t *out_words,
+                      size_t *out_count)
 {
-    if (!d || !d_in || !d_out || F == 0) return -1;
-    const int nch = d->nch;
+    const int nch = d->nch, T = d->ntracks;
     const size_t O = d->O;
     const int L = d->L;
-    const int S = dsdhip_das_plan(d, F, NULL, NULL);
-    std::vector<size_t> st((size_t)S), sz((size_t)S);
-    dsdhip_das_plan(d, F, st.data(), sz.data());
-    if (d->reposition && reposition(d, d_in, F, st, sz) != 0) return -1;
-    if (ensure_pool(d, S) != 0) return -1;
-    dsdhip_trellis_t *tr = d->tr;
     const int seg0_first = d->seg0_first;
-
-    /* segment inputs / keep rules */
-    std::vector<int64_t> in_start((size_t)S), in_cnt((size_t)S), keep((size_t)S);
-    for (int seg = 0; seg < S; seg++) {
-        const size_t ns = st[(size_t)seg], nz = sz[(size_t)seg];
-        if (seg == 0) {
-            size_t cnt = nz;
-            if (!seg0_first && S > 1 && nz + O <= F) cnt += O;
-            in_start[0] = 0;
-            in_cnt[0] = (int64_t)cnt;
-            keep[0] = 0;
-        } else if (seg == 1 && seg0_first) {
-            size_t cnt = nz;
-            if (S > 2 && ns + nz + O <= F) cnt += O;
-            in_start[1] = (int64_t)ns;
-            in_cnt[1] = (int64_t)cnt;
-            keep[1] = 0;
-        } else {
-            const bool ext = seg < S - 1;
-            const size_t is = ns >= O ? ns - O : 0;
-            size_t cnt = ns + nz - is;
-            if (ext && ns + nz + O <= F) cnt += O;
-            const size_t disc = ns >= O ? O - (size_t)L : 0;
-            in_start[(size_t)seg] = (int64_t)is;
-            in_cnt[(size_t)seg] = (int64_t)cnt;
-            keep[(size_t)seg] = disc ? (int64_t)L + (int64_t)disc : 0;
-        }
+    std::vector<TrackRun> R((size_t)T);
+    int Smax = 0;
+    for (int t = 0; t < T; t++) {
+        TrackRun &r = R[(size_t)t];
+        if (F[t] == 0) continue;
+        if (!d_in[t] || !d_out[t]) return -1;
+        r.in = d_in[t];
+        r.in_stride = in_stride[t];
+        r.F = F[t];
+        r.out = d_out[t];
+        r.out_words = out_words[t];
+        r.S = dsdhip_das_plan(d, r.F, NULL, NULL);
+        r.st.resize((size_t)r.S);
+        r.sz.resize((size_t)r.S);
+        dsdhip_das_plan(d, r.F, r.st.data(), r.sz.data());
+        if (d->reposition && reposition(d, r.in, r.F, r.st, r.sz) != 0) return -1;
+        if (r.S > Smax) Smax = r.S;
     }
-    /* packed outputs: one region per (ch, seg) */
-    std::vector<int64_t> woff((size_t)(nch * S));
+    if (Smax == 0) {
+        for (int t = 0; t < T; t++) if (out_count) out_count[t] = 0;
+        return 0;
+    }
+    if (ensure_pool(d, Smax) != 0) return -1;
+    dsdhip_trellis_t *tr = d->tr;
+
+    /* segment inputs / keep rules, packed output regions */
     int64_t wtot = 0;
-    for (int ch = 0; ch < nch; ch++)
+    for (int t = 0; t < T; t++) {
+        TrackRun &r = R[(size_t)t];
+        const int S = r.S;
+        if (S == 0) continue;
+        r.in_start.assign((size_t)S, 0);
+        r.in_cnt.assign((size_t)S, 0);
+        r.keep.assign((size_t)S, 0);
         for (int seg = 0; seg < S; seg++) {
-            woff[(size_t)(ch * S + seg)] = wtot;
-            wtot += (in_cnt[(size_t)seg] + 63) / 64 + 1;
+            const size_t ns = r.st[(size_t)seg], nz = r.sz[(size_t)seg];
+            if (seg == 0) {
+                size_t cnt = nz;
+                if (!seg0_first && S > 1 && nz + O <= r.F) cnt += O;
+                r.in_cnt[0] = (int64_t)cnt;
+            } else if (seg == 1 && seg0_first) {
+                size_t cnt = nz;
+                if (S > 2 && ns + nz + O <= r.F) cnt += O;
+                r.in_start[1] = (int64_t)ns;
+                r.in_cnt[1] = (int64_t)cnt;
+            } else {
+                const bool ext = seg < S - 1;
+                const size_t is = ns >= O ? ns - O : 0;
+                size_t cnt = ns + nz - is;
+                if (ext && ns + nz + O <= r.F) cnt += O;
+                const size_t disc = ns >= O ? O - (size_t)L : 0;
+                r.in_start[(size_t)seg] = (int64_t)is;
+                r.in_cnt[(size_t)seg] = (int64_t)cnt;
+                r.keep[(size_t)seg] = disc ? (int64_t)L + (int64_t)disc : 0;
+            }
         }
+        r.woff.assign((size_t)(nch * S), 0);
+        for (int ch = 0; ch < nch; ch++)
+            for (int seg = 0; seg < S; seg++) {
+                r.woff[(size_t)(ch * S + seg)] = wtot;
+                wtot += (r.in_cnt[(size_t)seg] + 63) / 64 + 1;
+            }
+        r.kept.assign((size_t)(nch * S), 0);
+        r.slow.assign((size_t)(nch * S), 0);
+    }
     DevBuf<unsigned long long> segbits;
     if (segbits.ensure((size_t)wtot)) return -1;
-    std::vector<int64_t> kept((size_t)(nch * S), 0), slow((size_t)(nch * S), 0);
 
-    auto run = [&](const std::vector<int> &segs, const std::vector<int> &srcs,
-                   const std::vector<double> &seeds) -> int {
-        const int nj = (int)segs.size();
+    struct JobRef { int t, ch, seg, src; };
+    std::vector<JobRef> jl;
+    std::vector<double> seeds;
+    auto add = [&](int t, int ch, int seg, int src, const double *seed) {
+        jl.push_back({t, ch, seg, src});
+        const double nan = nan_val();
+        for (int q = 0; q < 10; q++) seeds.push_back(seed ? seed[q] : nan);
+    };
+    auto run = [&]() -> int {
+        const int nj = (int)jl.size();
         if (nj == 0) return 0;
         std::vector<int> ctx((size_t)nj), cs((size_t)nj);
         std::vector<const double *> pin((size_t)nj);
         std::vector<uint64_t *> pout((size_t)nj);
         std::vector<int64_t> ni((size_t)nj), kf((size_t)nj), kp((size_t)nj), sl((size_t)nj);
         for (int j = 0; j < nj; j++) {
-            const int ch = segs[(size_t)j] / S, seg = segs[(size_t)j] % S;
-            ctx[(size_t)j] = (seg == 0) ? ix_persist(d, ch) : ix_seg(d, ch, seg);
-            cs[(size_t)j] = srcs[(size_t)j];
-            pin[(size_t)j] = d_in + (int64_t)ch * in_stride + in_start[(size_t)seg];
-            pout[(size_t)j] = (uint64_t *)(segbits.p + woff[(size_t)(ch * S + seg)]);
-            ni[(size_t)j] = in_cnt[(size_t)seg];
-            kf[(size_t)j] = keep[(size_t)seg];
+            const JobRef &q = jl[(size_t)j];
+            TrackRun &r = R[(size_t)q.t];
+            const int sidx = q.t * nch + q.ch;
+            ctx[(size_t)j] = (q.seg == 0) ? ix_persist(d, sidx) : ix_seg(d, sidx, q.seg);
+            cs[(size_t)j] = q.src;
+            pin[(size_t)j] = r.in + (int64_t)q.ch * r.in_stride + r.in_start[(size_t)q.seg];
+            pout[(size_t)j] = (uint64_t *)(segbits.p + r.woff[(size_t)(q.ch * r.S + q.seg)]);
+            ni[(size_t)j] = r.in_cnt[(size_t)q.seg];
+            kf[(size_t)j] = r.keep[(size_t)q.seg];
         }
         int err = 0;
-        int rc = dsdhip_trellis_run_jobs(tr, nj, ctx.data(), cs.data(), pin.data(), ni.data(),
-                                         kf.data(), pout.data(), seeds.data(), kp.data(),
-                                         sl.data(), &err);
+        const int rc = dsdhip_trellis_run_jobs(tr, nj, ctx.data(), cs.data(), pin.data(), ni.data(),
+                                               kf.data(), pout.data(), seeds.data(), kp.data(),
+                                               sl.data(), &err);
         for (int j = 0; j < nj; j++) {
-            kept[(size_t)segs[(size_t)j]] = kp[(size_t)j];
-            slow[(size_t)segs[(size_t)j]] = sl[(size_t)j];
+            const JobRef &q = jl[(size_t)j];
+            TrackRun &r = R[(size_t)q.t];
+            r.kept[(size_t)(q.ch * r.S + q.seg)] = kp[(size_t)j];
+            r.slow[(size_t)(q.ch * r.S + q.seg)] = sl[(size_t)j];
         }
+        jl.clear();
+        seeds.clear();
         return rc;
     };
-    const double nan = nan_val();
-    std::vector<int> segs, srcs;
-    std::vector<double> seeds;
-    auto add = [&](int ch, int seg, int src, const double *seed) {
-        segs.push_back(ch * S + seg);
-        srcs.push_back(src);
-        for (int q = 0; q < 10; q++) seeds.push_back(seed ? seed[q] : nan);
-    };
+
     if (seg0_first) {
-        for (int ch = 0; ch < nch; ch++) add(ch, 0, -1, NULL);
-        if (run(segs, srcs, seeds) != 0) return -1;
-        segs.clear(); srcs.clear(); seeds.clear();
-        std::vector<double> est;
-        if (S > 2) {
+        for (int t = 0; t < T; t++)
+            for (int ch = 0; ch < nch && R[(size_t)t].S > 0; ch++) add(t, ch, 0, -1, NULL);
+        if (run() != 0) return -1;
+        std::vector<std::vector<double>> est((size_t)T);
+        tr_params prm;
+        dsdhip_trellis_params(tr, &prm);
+        for (int t = 0; t < T; t++) {
+            TrackRun &r = R[(size_t)t];
+            const int S = r.S;
+            if (S <= 2) continue;
             /* estimate from seg1's start over the rest (src/threadpool.c:345-366) */
             std::vector<double> init((size_t)nch * 10);
             for (int ch = 0; ch < nch; ch++)
-                if (dsdhip_trellis_best_state(tr, ix_persist(d, ch), &init[(size_t)ch * 10]) != 0) return -1;
-            const int64_t es = (int64_t)st[1];
+                if (dsdhip_trellis_best_state(tr, ix_persist(d, t * nch + ch), &init[(size_t)ch * 10]) != 0)
+                    return -1;
+            const int64_t es = (int64_t)r.st[1];
             std::vector<int64_t> bnd;
-            for (int seg = 1; seg < S; seg++) bnd.push_back(seg >= 2 ? (int64_t)st[(size_t)seg] - es : 0);
-            tr_params prm;
-            dsdhip_trellis_params(tr, &prm);
+            for (int seg = 1; seg < S; seg++) bnd.push_back(seg >= 2 ? (int64_t)r.st[(size_t)seg] - es : 0);
             DevBuf<int64_t> db;
             DevBuf<double> dinit, dest, dab;
             if (db.ensure(bnd.size()) || dinit.ensure(init.size()) ||
@@ -483,51 +555,71 @@ int dsdhip_das_run_dev(dsdhip_das_t *d, const double *d_in, int64_t in_stride, s
             hipMemcpyAsync(db.p, bnd.data(), sizeof(int64_t) * bnd.size(), hipMemcpyHostToDevice, d->stream);
             hipMemcpyAsync(dinit.p, init.data(), sizeof(double) * init.size(), hipMemcpyHostToDevice, d->stream);
             hipMemcpyAsync(dab.p, ab, sizeof(ab), hipMemcpyHostToDevice, d->stream);
-            hipLaunchKernelGGL(das_estimate, dim3((unsigned)((nch + 63) / 64)), dim3(64), 0, d->stream, d_in, in_stride, es,
-                               (int64_t)F - es, db.p, (int)bnd.size(), dinit.p, dab.p,
-                               dab.p + TR_MAXP, prm.P, prm.state_limit, 16, nch, dest.p);
-            est.resize((size_t)nch * bnd.size() * 10);
-            hipMemcpyAsync(est.data(), dest.p, sizeof(double) * est.size(), hipMemcpyDeviceToHost, d->stream);
+            hipLaunchKernelGGL(das_estimate, dim3((unsigned)((nch + 63) / 64)), dim3(64), 0, d->stream, r.in,
+                               r.in_stride, es, (int64_t)r.F - es, db.p, (int)bnd.size(), dinit.p,
+                               dab.p, dab.p + TR_MAXP, prm.P, prm.state_limit, 16, nch, dest.p);
+            est[(size_t)t].resize((size_t)nch * bnd.size() * 10);
+            hipMemcpyAsync(est[(size_t)t].data(), dest.p, sizeof(double) * est[(size_t)t].size(),
+                           hipMemcpyDeviceToHost, d->stream);
             if (hipStreamSynchronize(d->stream) != hipSuccess) return -1;
         }
-        const int nb = S - 1;
-        for (int ch = 0; ch < nch; ch++) {
-            if (S > 1) add(ch, 1, ix_persist(d, ch), NULL);
-            for (int seg = 2; seg < S; seg++)
-                add(ch, seg, ix_persist(d, ch), &est[((size_t)ch * nb + (size_t)(seg - 1)) * 10]);
+        for (int t = 0; t < T; t++) {
+            TrackRun &r = R[(size_t)t];
+            const int S = r.S;
+            const int nb = S - 1;
+            for (int ch = 0; ch < nch && S > 0; ch++) {
+                const int sidx = t * nch + ch;
+                if (S > 1) add(t, ch, 1, ix_persist(d, sidx), NULL);
+                for (int seg = 2; seg < S; seg++)
+                    add(t, ch, seg, ix_persist(d, sidx),
+                        &est[(size_t)t][((size_t)ch * nb + (size_t)(seg - 1)) * 10]);
+            }
         }
-        if (run(segs, srcs, seeds) != 0) return -1;
+        if (run() != 0) return -1;
     } else {
-        std::vector<double> pst((size_t)nch * 10);
-        for (int ch = 0; ch < nch; ch++) {
-            if (dsdhip_trellis_best_state(tr, ix_persist(d, ch), &pst[(size_t)ch * 10]) != 0) return -1;
-            if (dsdhip_trellis_clone(tr, ix_snap(d, ch), ix_persist(d, ch)) != 0) return -1;
-        }
-        for (int ch = 0; ch < nch; ch++) {
-            add(ch, 0, -1, NULL);
-            for (int seg = 1; seg < S; seg++) add(ch, seg, ix_snap(d, ch), &pst[(size_t)ch * 10]);
-        }
-        if (run(segs, srcs, seeds) != 0) return -1;
+        std::vector<double> pst((size_t)d->ns * 10);
+        for (int t = 0; t < T; t++)
+            for (int ch = 0; ch < nch && R[(size_t)t].S > 0; ch++) {
+                const int sidx = t * nch + ch;
+                if (dsdhip_trellis_best_state(tr, ix_persist(d, sidx), &pst[(size_t)sidx * 10]) != 0) return -1;
+                if (dsdhip_trellis_clone(tr, ix_snap(d, sidx), ix_persist(d, sidx)) != 0) return -1;
+            }
+        for (int t = 0; t < T; t++)
+            for (int ch = 0; ch < nch && R[(size_t)t].S > 0; ch++) {
+                const int sidx = t * nch + ch;
+                add(t, ch, 0, -1, NULL);
+                for (int seg = 1; seg < R[(size_t)t].S; seg++)
+                    add(t, ch, seg, ix_snap(d, sidx), &pst[(size_t)sidx * 10]);
+            }
+        if (run() != 0) return -1;
     }
     int64_t tslow = 0;
-    for (size_t i = 0; i < slow.size(); i++) tslow += slow[i];
+    for (int t = 0; t < T; t++)
+        for (size_t i = 0; i < R[(size_t)t].slow.size(); i++) tslow += R[(size_t)t].slow[i];
     d->last_slow = tslow;
 
-    /* ---- stitch on channel 0 ---- */
+    /* ---- stitch on channel 0 of every track, all boundaries in one launch ---- */
     const int first = seg0_first ? 2 : 1;
-    std::vector<int> bp((size_t)S, 0);
-    int64_t s0 = kept[0];
-    if (!seg0_first && s0 > (int64_t)sz[0]) s0 = (int64_t)sz[0];
-    const int nbd = S - first > 0 ? S - first : 0;
-    std::vector<int64_t> pbit((size_t)nbd > 0 ? (size_t)nbd : 1), cbit((size_t)nbd > 0 ? (size_t)nbd : 1);
-    std::vector<int> ovl((size_t)nbd > 0 ? (size_t)nbd : 1);
-    for (int k = first; k < S; k++) {
-        const int64_t prev_end = (k - 1 == 0) ? s0 : kept[(size_t)k - 1];
-        if (prev_end < 2 * (int64_t)O || kept[(size_t)k] == 0) return -1;   /* plan too fine for span-local stitching */
-        pbit[(size_t)(k - first)] = woff[(size_t)(k - 1)] * 64 + prev_end - (int64_t)O;
-        cbit[(size_t)(k - first)] = woff[(size_t)k] * 64;
-        ovl[(size_t)(k - first)] = (int)((int64_t)O < kept[(size_t)k] ? (int64_t)O : kept[(size_t)k]);
+    std::vector<int64_t> pbit, cbit;
+    std::vector<int> ovl, bt, bk;           /* boundary -> (track, segment) */
+    for (int t = 0; t < T; t++) {
+        TrackRun &r = R[(size_t)t];
+        const int S = r.S;
+        r.bp.assign((size_t)(S > 0 ? S : 1), 0);
+        if (S == 0) continue;
+        int64_t s0 = r.kept[0];
+        if (!seg0_first && s0 > (int64_t)r.sz[0]) s0 = (int64_t)r.sz[0];
+        for (int k = first; k < S; k++) {
+            const int64_t prev_end = (k - 1 == 0) ? s0 : r.kept[(size_t)k - 1];
+            if (prev_end < 2 * (int64_t)O || r.kept[(size_t)k] == 0) return -1;   /* plan too fine for span-local stitching */
+            pbit.push_back(r.woff[(size_t)(k - 1)] * 64 + prev_end - (int64_t)O);
+            cbit.push_back(r.woff[(size_t)k] * 64);
+            ovl.push_back((int)((int64_t)O < r.kept[(size_t)k] ? (int64_t)O : r.kept[(size_t)k]));
+            bt.push_back(t);
+            bk.push_back(k);
+        }
     }
+    const int nbd = (int)pbit.size();
     if (nbd > 0) {
         DevBuf<int64_t> dpb, dcb;
         DevBuf<int> dov, dnc, dpk, dpos, dden;
@@ -566,67 +658,98 @@ int dsdhip_das_run_dev(dsdhip_das_t *d, const double *d_in, int64_t in_stride, s
                 if (rq < 0.8) best = c;
                 else if (rq < 1.2 && den[(size_t)i * QN + c] > den[(size_t)i * QN + best]) best = c;
             }
-            bp[(size_t)(i + first)] = pos[(size_t)i * QN + best];
+            R[(size_t)bt[(size_t)i]].bp[(size_t)bk[(size_t)i]] = pos[(size_t)i * QN + best];
         }
     }
 
-    /* ---- assemble (same positions on every channel) ---- */
-    std::vector<int64_t> sbit, dbit, len;
-    int64_t total = 0;
-    for (int ch = 0; ch < nch; ch++) {
-        std::vector<int64_t> at((size_t)S), src((size_t)S), cc((size_t)S);
-        int64_t c0 = kept[(size_t)(ch * S)];
-        if (!seg0_first && c0 > (int64_t)sz[0]) c0 = (int64_t)sz[0];
-        at[0] = 0;
-        src[0] = woff[(size_t)(ch * S)] * 64;
-        cc[0] = c0;
-        int64_t wp = c0;
-        for (int k = 1; k < S; k++) {
-            const int64_t kk = kept[(size_t)(ch * S + k)];
-            if (kk == 0) { at[(size_t)k] = wp; src[(size_t)k] = 0; cc[(size_t)k] = 0; continue; }
-            int64_t a, b;
-            if (k == 1 && seg0_first) { a = wp; b = 0; }
-            else {
-                const int64_t pst = wp >= (int64_t)O ? wp - (int64_t)O : 0;
-                b = bp[(size_t)k];
-                a = pst + b;
+    /* ---- assemble each track (same positions on every channel) ---- */
+    std::vector<DevBuf<int64_t>> dsb((size_t)T), ddb((size_t)T), dlb((size_t)T);
+    for (int t = 0; t < T; t++) {
+        TrackRun &r = R[(size_t)t];
+        const int S = r.S;
+        if (S == 0) continue;
+        std::vector<int64_t> sbit, dbit, len;
+        int64_t total = 0;
+        for (int ch = 0; ch < nch; ch++) {
+            std::vector<int64_t> at((size_t)S), src((size_t)S), cc((size_t)S);
+            int64_t c0 = r.kept[(size_t)(ch * S)];
+            if (!seg0_first && c0 > (int64_t)r.sz[0]) c0 = (int64_t)r.sz[0];
+            at[0] = 0;
+            src[0] = r.woff[(size_t)(ch * S)] * 64;
+            cc[0] = c0;
+            int64_t wp = c0;
+            for (int k = 1; k < S; k++) {
+                const int64_t kk = r.kept[(size_t)(ch * S + k)];
+                if (kk == 0) { at[(size_t)k] = wp; src[(size_t)k] = 0; cc[(size_t)k] = 0; continue; }
+                int64_t a, b;
+                if (k == 1 && seg0_first) { a = wp; b = 0; }
+                else {
+                    const int64_t pst = wp >= (int64_t)O ? wp - (int64_t)O : 0;
+                    b = r.bp[(size_t)k];
+                    a = pst + b;
+                }
+                at[(size_t)k] = a;
+                src[(size_t)k] = r.woff[(size_t)(ch * S + k)] * 64 + b;
+                cc[(size_t)k] = kk - b;
+                wp = a + (kk - b);
             }
-            at[(size_t)k] = a;
-            src[(size_t)k] = woff[(size_t)(ch * S + k)] * 64 + b;
-            cc[(size_t)k] = kk - b;
-            wp = a + (kk - b);
+            if (ch == 0) total = wp;
+            for (int k = 0; k < S; k++) {
+                /* effective span: up to where the next non-empty span starts */
+                int64_t e = at[(size_t)k] + cc[(size_t)k];
+                for (int k2 = k + 1; k2 < S; k2++)
+                    if (cc[(size_t)k2] > 0) { if (at[(size_t)k2] < e) e = at[(size_t)k2]; break; }
+                sbit.push_back(src[(size_t)k]);
+                dbit.push_back(at[(size_t)k]);
+                len.push_back(e > at[(size_t)k] ? e - at[(size_t)k] : 0);
+            }
         }
-        if (ch == 0) total = wp;
-        for (int k = 0; k < S; k++) {
-            /* effective span: up to where the next non-empty span starts */
-            int64_t e = at[(size_t)k] + cc[(size_t)k];
-            for (int k2 = k + 1; k2 < S; k2++)
-                if (cc[(size_t)k2] > 0) { if (at[(size_t)k2] < e) e = at[(size_t)k2]; break; }
-            sbit.push_back(src[(size_t)k]);
-            dbit.push_back(at[(size_t)k]);
-            len.push_back(e > at[(size_t)k] ? e - at[(size_t)k] : 0);
-        }
-    }
-    if ((total + 63) / 64 > out_words) return -1;
-    {
-        DevBuf<int64_t> ds, dd, dl;
-        if (ds.ensure(sbit.size()) || dd.ensure(dbit.size()) || dl.ensure(len.size())) return -1;
-        hipMemcpyAsync(ds.p, sbit.data(), sizeof(int64_t) * sbit.size(), hipMemcpyHostToDevice, d->stream);
-        hipMemcpyAsync(dd.p, dbit.data(), sizeof(int64_t) * dbit.size(), hipMemcpyHostToDevice, d->stream);
-        hipMemcpyAsync(dl.p, len.data(), sizeof(int64_t) * len.size(), hipMemcpyHostToDevice, d->stream);
-        const int64_t words = out_words;
+        if ((total + 63) / 64 > r.out_words) return -1;
+        r.total = total;
+        if (dsb[(size_t)t].ensure(sbit.size()) || ddb[(size_t)t].ensure(dbit.size()) ||
+            dlb[(size_t)t].ensure(len.size()))
+            return -1;
+        hipMemcpyAsync(dsb[(size_t)t].p, sbit.data(), sizeof(int64_t) * sbit.size(), hipMemcpyHostToDevice, d->stream);
+        hipMemcpyAsync(ddb[(size_t)t].p, dbit.data(), sizeof(int64_t) * dbit.size(), hipMemcpyHostToDevice, d->stream);
+        hipMemcpyAsync(dlb[(size_t)t].p, len.data(), sizeof(int64_t) * len.size(), hipMemcpyHostToDevice, d->stream);
+        const int64_t words = r.out_words;
         hipLaunchKernelGGL(das_assemble, dim3((unsigned)((words + 255) / 256), (unsigned)nch), dim3(256), 0,
-                           d->stream, segbits.p, ds.p, dd.p, dl.p, S, d_out, words, total);
-        if (hipStreamSynchronize(d->stream) != hipSuccess) return -1;
+                           d->stream, segbits.p, dsb[(size_t)t].p, ddb[(size_t)t].p, dlb[(size_t)t].p, S,
+                           r.out, words, total);
     }
+    if (hipStreamSynchronize(d->stream) != hipSuccess) return -1;
     /* ---- copy-back ---- */
-    if (S > 1)
-        for (int ch = 0; ch < nch; ch++)
-            if (dsdhip_trellis_copy_detached(tr, ix_persist(d, ch), ix_seg(d, ch, S - 1)) != 0) return -1;
-    d->last_S = S;
-    d->last_stitch = bp;
-    if (out_count) *out_count = (size_t)total;
+    for (int t = 0; t < T; t++) {
+        const int S = R[(size_t)t].S;
+        if (S > 1)
+            for (int ch = 0; ch < nch; ch++) {
+                const int sidx = t * nch + ch;
+                if (dsdhip_trellis_copy_detached(tr, ix_persist(d, sidx), ix_seg(d, sidx, S - 1)) != 0)
+                    return -1;
+            }
+        if (out_count) out_count[t] = (size_t)R[(size_t)t].total;
+    }
+    d->last_S = R[0].S;
+    d->last_stitch = R[0].bp;
     return 0;
+}
+
+int dsdhip_das_run_dev(dsdhip_das_t *d, const double *d_in, int64_t in_stride, size_t F,
+                       unsigned long long *d_out, int64_t out_words, size_t *out_count)
+{
+    if (!d || !d_in || !d_out || F == 0 || d->ntracks != 1) return -1;
+    return run_tracks(d, &d_in, &in_stride, &F, &d_out, &out_words, out_count);
+}
+
+/* Every track of a multi-track object: d_in[t] = [nch][in_stride[t]] doubles
+ * with F[t] samples (0: the track is not run), d_out[t] = [nch][out_words[t]]
+ * packed bits; out_count[t] receives the bits produced per channel. */
+int dsdhip_das_run_tracks_dev(dsdhip_das_t *d, const double *const *d_in, const int64_t *in_stride,
+                              const size_t *F, unsigned long long *const *d_out,
+                              const int64_t *out_words, size_t *out_count)
+{
+    if (!d || !d_in || !in_stride || !F || !d_out || !out_words) return -1;
+    return run_tracks(d, d_in, in_stride, F, d_out, out_words, out_count);
 }
 
 int dsdhip_das_last_info(const dsdhip_das_t *d, int *nseg, int64_t *slow_steps, int *stitch_pos,

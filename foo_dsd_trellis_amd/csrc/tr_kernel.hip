@@ -664,3 +664,20 @@ extern "C" int tr_launch_queue(const tr_params *prm, tr_job *d_jobs, int njobs, 
     TQ_CASE_P(4) TQ_CASE_P(5) TQ_CASE_P(6) TQ_CASE_P(7) TQ_CASE_P(8)
     return -1;
 }
+
+/* Resident workgroups (one wave each) per CU of tr_run_jobs for prm. */
+extern "C" int tr_occupancy(const tr_params *prm)
+{
+    const int P = prm->P;
+    const int hw_need = (prm->L + 63) / 64;
+    int HW = 1;
+    while (HW < hw_need) HW *= 2;
+    int n = 0;
+#define TO_CASE(PP, HH) \
+    if (P == PP && HW == HH) { \
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)tr_run_jobs<PP, HH>, 64, 0) != hipSuccess) return -1; \
+        return n; }
+#define TO_CASE_P(PP) TO_CASE(PP, 1) TO_CASE(PP, 2) TO_CASE(PP, 4) TO_CASE(PP, 8) TO_CASE(PP, 16) TO_CASE(PP, 32)
+    TO_CASE_P(4) TO_CASE_P(5) TO_CASE_P(6) TO_CASE_P(7) TO_CASE_P(8)
+    return -1;
+}

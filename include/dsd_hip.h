@@ -204,6 +204,17 @@ int dsdhip_das_reset(dsdhip_das_t *d);
 /* HIP-event timing of the segment trellis launches (see dsdhip_trellis_timing). */
 int dsdhip_das_timing(dsdhip_das_t *d, int on);
 int dsdhip_das_timing_read(dsdhip_das_t *d, double *ms, int64_t *launches, int64_t *samples, int clear);
+/* Batched form: ntracks tracks of nch channels each, every track with its
+ * own persistent contexts (no reference counterpart; the reference runs one
+ * DAS per plugin instance, src/dsp_plugin.c:1721-2761).  One call runs the
+ * next chunk of every track with F[t] > 0: all segments of all tracks in
+ * one trellis launch, all stitch boundaries in one launch. */
+dsdhip_das_t *dsdhip_das_create_tracks(int ntf_index, int depth, int cands, int lat,
+                                       double state_limit, int nch, int ntracks, size_t seg_len,
+                                       int nseg, int seg0_first, int reposition);
+int dsdhip_das_run_tracks_dev(dsdhip_das_t *d, const double *const *d_in, const int64_t *in_stride,
+                              const size_t *F, unsigned long long *const *d_out,
+                              const int64_t *out_words, size_t *out_count);
 /* Segment count, literal-prune steps and stitch positions of the last run. */
 int dsdhip_das_last_info(const dsdhip_das_t *d, int *nseg, int64_t *slow_steps, int *stitch_pos,
                          int max_pos);

@@ -1,15 +1,16 @@
 """Batched DAS (gpu_hip_plan_*): tracks of uneven length, chunk-streamed,
-one persistent trellis launch per chunk, hipGraph replay, against the
-oracle's per-track FIR chain and per-chunk DAS with persistent contexts."""
+all segments of all tracks in one trellis launch per chunk, against the
+oracle's per-track FIR chain and per-chunk DAS with persistent contexts.
+The device-side graph-captured chunk step is disabled in this build (its
+first GPU run faulted, cause not found); plans run the host-orchestrated
+batched step, which these tests exercise."""
 import numpy as np
 import pytest
 import torch
 
 import oracle_lib as ol
 
-pytestmark = [pytest.mark.gpu, pytest.mark.skip(
-    reason="batched plan launch disabled: its first GPU run ended in a memory fault whose "
-           "cause is not yet found; gpu_hip_plan_launch_graph refuses to run")]
+pytestmark = pytest.mark.gpu
 
 FS_IN, FS_OUT = 44100, 5644800          # x128, 7 half-band stages
 K, L, DEPTH = 16, 32, 8
@@ -47,13 +48,12 @@ def gpu_bits(plan, t, nbits):
     return np.where(bits > 0, 1.0, -1.0).astype(np.float32)
 
 
-@pytest.mark.parametrize("use_graph", [True, False])
-def test_batch_matches_oracle(gpu, oracle, use_graph):
+def test_batch_matches_oracle(gpu, oracle):
     from foo_dsd_trellis_amd import batch, trellis
     fi = trellis.ntf_auto(FS_OUT)
     pcm = tracks()
     plan = batch.BatchPlan(FS_IN, FS_OUT, fi, K=K, L=L, depth=DEPTH, nch=2, chunk_frames=CF,
-                           seg_len=8192, waves_per_group=2, waves_per_cu=4, use_graph=use_graph)
+                           seg_len=8192, use_graph=False)
     plan.enqueue([torch.from_numpy(x).cuda() for x in pcm])
     first = plan.run(max_chunks=1)
     bits = plan.run()
@@ -78,7 +78,8 @@ def test_batch_autotuned(gpu, oracle):
     tune = plan.tune()
     print("tuned:", tune)
     assert tune["seg_len"] in (16384, 32768, 65536, 131072)
-    assert tune["waves_per_group"] in (1, 2, 4) and tune["trial_msps"] > 0
+    assert tune["waves_per_group"] == 1 and tune["waves_per_cu"] >= 1 and tune["trial_msps"] > 0
+    assert tune["use_graph"] is False          # the graph-captured step is disabled in this build
     for t, x in enumerate(pcm):
         ref = oracle_bits(oracle, x, fi, tune["seg_len"])
         assert np.array_equal(gpu_bits(plan, t, int(bits[t])), ref), "track %d" % t
