@@ -150,6 +150,66 @@ def diff_loop_filter(steps, check_rows):
     return out
 
 
+def batched_uneven(chunks, cf, plain_tracks):
+    """The batched uneven workload, bounded: 16 tracks of 6-channel 48 kHz
+    PCM (lengths drawn from U[5, 120] s, seeded band-limited Gaussian at
+    -14 dBFS RMS) -> 9 half-band stages to DSD512/48 (24.576 MHz) -> DAS
+    trellis K=32, L=64 with every segment of every channel and track in one
+    launch per chunk.  The first `chunks` chunks of `cf` frames are run (every
+    track is longer than that); the first chunk (tuning, set-up) is not
+    timed.  `plain_tracks` tracks are also run one DAS launch per track on
+    the same first chunk, for the per-track form's throughput."""
+    import torch
+    from foo_dsd_trellis_amd import batch, das, fir, signals, trellis
+    fs_in, fs_out, nch = 48000, 24576000, 6
+    R = fs_out // fs_in
+    lens = signals.uneven_tracks(16, nch, fs_in, seed=4)
+    n = chunks * cf
+    pcm = [torch.from_numpy(signals.bandlimited_gauss(min(L_, n), fs=fs_in, nch=nch, seed=100 + t)).cuda()
+           for t, L_ in enumerate(lens)]
+    fi = trellis.ntf_auto(fs_out)
+    plan = batch.BatchPlan(fs_in, fs_out, fi, K=K, L=L, depth=B, nch=nch, chunk_frames=cf)
+    plan.enqueue(pcm)
+    plan.run(max_chunks=1)
+    torch.cuda.synchronize()
+    b0 = plan.run(max_chunks=0)
+    t0 = time.perf_counter()
+    b1 = plan.run(max_chunks=chunks - 1)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out_bits = int((b1 - b0).sum()) * nch
+    tune = plan.tune()
+    res = {"value": round(out_bits / dt / 1e6, 3), "unit": "Msamples/s",
+           "x_realtime_per_track": round((chunks - 1) * cf / fs_in / dt * len(pcm), 3),
+           "timed_chunks": chunks - 1, "chunk_frames": cf, "tracks": len(pcm), "channels": nch,
+           "track_seconds": [round(L_ / fs_in, 2) for L_ in lens], "tune": tune,
+           "mode": "host-orchestrated batched step (device-side graph step disabled in this build)",
+           "config": "16 tracks x 6 ch 48 kHz f32 -> DSD512/48 (24.576 MHz, 9 stages) -> DAS K=32 L=64 B=8, "
+                     "all segments of all tracks in one trellis launch per chunk"}
+    if plain_tracks > 0:
+        # per-track form on the first chunk of plain_tracks tracks: FIR, then one DAS per track
+        ch = fir.FirChain(fs_in, fs_out, fir.PCM_FIR_GAIN, nch)
+        d_f = torch.empty((nch, cf * R), dtype=torch.float64, device="cuda")
+        words = cf * R // 64 + 2
+        d_o = torch.zeros((nch, words), dtype=torch.int64, device="cuda")
+        tot, tt = 0, 0.0
+        for t in range(plain_tracks):
+            x = pcm[t][:, :cf].contiguous()
+            ch.reset()
+            ch.process_dev(x.data_ptr(), cf, cf, d_f.data_ptr(), cf * R)
+            D = das.Das(fi, B, K, L, nch, seg_len=tune["seg_len"], seg0_first=False, reposition=True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            nout = D.run_dev(d_f.data_ptr(), cf * R, cf * R, d_o.data_ptr(), words)
+            torch.cuda.synchronize()
+            tt += time.perf_counter() - t0
+            tot += nout * nch
+        res["plain_per_track"] = {"value": round(tot / tt / 1e6, 3), "unit": "Msamples/s",
+                                  "sample": "first chunk of %d tracks, one DAS launch per track "
+                                            "(same segment length)" % plain_tracks}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +225,10 @@ def main():
     ap.add_argument("--lf-steps", type=int, default=5,
                     help="differentiable loop filter fwd+bwd steps (0: skip)")
     ap.add_argument("--lf-check-rows", type=int, default=2)
+    ap.add_argument("--batch-chunks", type=int, default=3,
+                    help="chunks of the batched uneven workload (first untimed; 0: skip)")
+    ap.add_argument("--batch-chunk-frames", type=int, default=12000)
+    ap.add_argument("--batch-plain-tracks", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
@@ -245,6 +309,8 @@ def main():
         cpu = cpu_baseline(np.ascontiguousarray(seg), fi, thr, args.cpu_seconds)
 
     lf = diff_loop_filter(args.lf_steps, args.lf_check_rows) if args.lf_steps > 0 else None
+    bu = (batched_uneven(args.batch_chunks, args.batch_chunk_frames, args.batch_plain_tracks)
+          if args.batch_chunks > 1 else None)
 
     line = {
         "metric": "Trellis SDM Msamples/s & xrealtime at DSD256 K=32, batched vs plain",
@@ -262,6 +328,7 @@ def main():
         "plain_gpu": plain,
         "cpu_baseline": cpu,
         "diff_loop_filter": lf,
+        "batched_uneven": bu,
     }
     print(json.dumps(line), flush=True)
 
