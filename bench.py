@@ -210,6 +210,48 @@ def batched_uneven(chunks, cf, plain_tracks):
     return res
 
 
+def streaming_room(chunks, seg_len):
+    """The streaming workload: 6-channel 44.1 kHz float32 PCM (seeded pink
+    noise plus tones, as the headline workload, one seed per channel) in
+    chunks of 8192 frames -> eight half-band stages to DSD256 (2,097,152
+    samples per chunk per channel) -> UPOLS convolution with a seeded
+    synthetic room IR per channel (2^20 taps, exponentially decaying Gaussian
+    noise, RT60 0.4 s, unit energy; partitions of 8192, 128 partitions, FFT
+    16384, complex FP32 spectra) -> DAS trellis K=32, L=64, B=8 with all
+    segments of all channels in one launch.  Chunk latency is the wall time
+    of one chunk call (FIR, convolution, DAS, bits in HBM); the first chunk
+    is not timed."""
+    import torch
+    from foo_dsd_trellis_amd import signals, stream, trellis
+    nch, cf = 6, 8192
+    ir = signals.room_ir(1 << 20, fs=DSD256, nch=nch, rt60=0.4, seed=5)
+    fi = trellis.ntf_auto(DSD256)
+    S = stream.Stream(FS_IN, DSD256, nch, cf, ir=ir, P=8192, ntf_index=fi, K=K, L=L, depth=B,
+                      seg_len=seg_len)
+    pcm = signals.pink_multitone(cf * (chunks + 1), fs=FS_IN, nch=nch, seed=7)
+    d_pcm = torch.from_numpy(np.ascontiguousarray(pcm)).cuda()
+    torch.cuda.synchronize()
+    S.process(d_pcm[:, :cf])
+    torch.cuda.synchronize()
+    lat, tot = [], 0
+    for c in range(1, chunks + 1):
+        t0 = time.perf_counter()
+        tot += S.process(d_pcm[:, c * cf:(c + 1) * cf])
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+    lat_ms = np.array(lat) * 1000.0
+    wall = float(np.sum(lat))
+    return {"value": round(tot * nch / wall / 1e6, 3), "unit": "Msamples/s",
+            "x_realtime": round(chunks * cf / FS_IN / wall, 3),
+            "chunk_ms_p50": round(float(np.percentile(lat_ms, 50)), 3),
+            "chunk_ms_p99": round(float(np.percentile(lat_ms, 99)), 3),
+            "chunk_ms_max": round(float(lat_ms.max()), 3),
+            "chunk_audio_ms": round(cf / FS_IN * 1000.0, 3), "timed_chunks": chunks,
+            "seg_len": seg_len,
+            "config": "6 ch 44.1 kHz f32, 8192-frame chunks -> DSD256 (8 stages) -> UPOLS 2^20-tap "
+                      "room IR (P=8192, c-f32) -> DAS K=32 L=64 B=8, one stream, chunks in order"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -229,6 +271,9 @@ def main():
                     help="chunks of the batched uneven workload (first untimed; 0: skip)")
     ap.add_argument("--batch-chunk-frames", type=int, default=12000)
     ap.add_argument("--batch-plain-tracks", type=int, default=2)
+    ap.add_argument("--stream-chunks", type=int, default=30,
+                    help="timed chunks of the streaming room-correction workload (0: skip)")
+    ap.add_argument("--stream-seg-len", type=int, default=8192)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
@@ -311,6 +356,7 @@ def main():
     lf = diff_loop_filter(args.lf_steps, args.lf_check_rows) if args.lf_steps > 0 else None
     bu = (batched_uneven(args.batch_chunks, args.batch_chunk_frames, args.batch_plain_tracks)
           if args.batch_chunks > 1 else None)
+    sr = streaming_room(args.stream_chunks, args.stream_seg_len) if args.stream_chunks > 0 else None
 
     line = {
         "metric": "Trellis SDM Msamples/s & xrealtime at DSD256 K=32, batched vs plain",
@@ -329,6 +375,7 @@ def main():
         "cpu_baseline": cpu,
         "diff_loop_filter": lf,
         "batched_uneven": bu,
+        "streaming_room": sr,
     }
     print(json.dumps(line), flush=True)
 

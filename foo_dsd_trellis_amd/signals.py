@@ -75,3 +75,17 @@ def multitone_noise(nrows, n, fs=5644800, seed=9, dbfs=-10.0, ntones=5, fmin=200
         x = x + np.sqrt(0.5) * rms * rng.standard_normal(n)
         out[r] = x.astype(np.float32)
     return out
+
+
+def room_ir(taps=1 << 20, fs=11289600, nch=6, rt60=0.4, seed=5):
+    """Seeded synthetic room impulse responses (nch, taps) float64:
+    Gaussian noise with an exponential decay of 60 dB over rt60 seconds,
+    scaled to unit energy per channel."""
+    rng = np.random.default_rng(seed)
+    t = np.arange(taps) / fs
+    env = np.exp(-np.log(1000.0) * t / rt60)
+    out = np.empty((nch, taps))
+    for c in range(nch):
+        h = rng.standard_normal(taps) * env
+        out[c] = h / np.sqrt(np.sum(h * h))
+    return out

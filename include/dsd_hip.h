@@ -267,6 +267,25 @@ int gpu_hip_plan_info(const gpu_hip_plan_t *p, int64_t *chunks_total, int64_t *c
 void gpu_hip_plan_destroy(gpu_hip_plan_t *p);
 
 /* ======================================================================
+ * 8. UPOLS convolution (uniformly partitioned overlap-save), replaces
+ *    conv_init / conv_load_ir / conv_process (src/convolution.c:530-1041)
+ *    for a given IR per channel at the signal rate: partitions of P
+ *    samples, FFT size 2P, complex FP32 spectra.  Input and output are
+ *    device f64 [nch][stride]; chunks are multiples of 8*P samples.
+ * ====================================================================== */
+typedef struct dsdhip_conv dsdhip_conv_t;
+/* ir: host f64 [nch][ir_stride], ir_len taps per channel. */
+dsdhip_conv_t *dsdhip_conv_create(int nch, int P, const double *ir, int64_t ir_len, int64_t ir_stride);
+void dsdhip_conv_destroy(dsdhip_conv_t *c);
+/* replaces conv_reset (src/convolution.c:1036): clears the delay line. */
+int dsdhip_conv_reset(dsdhip_conv_t *c);
+int dsdhip_conv_set_stream(dsdhip_conv_t *c, void *hip_stream);
+int dsdhip_conv_partitions(const dsdhip_conv_t *c);
+/* y = x * h for the next n samples of every channel (stream-ordered). */
+int dsdhip_conv_process_dev(dsdhip_conv_t *c, const double *d_in, int64_t in_stride, size_t n,
+                            double *d_out, int64_t out_stride);
+
+/* ======================================================================
  * 7. Differentiable loop filter and pre-emphasis (training).
  *    Replaces the per-sample torch loop of training_pipeline/diff_sdm.py:
  *    61-100 (DiffSDM.forward) and the conv1d of model_pre_sdm.py:44-53
@@ -300,6 +319,37 @@ int dsdhip_pe_forward(const double *d_w, int ntaps, const float *d_p, int B, siz
 /* Tap gradient gw[k] = sum_{b,n} gu[n][b] p[b][n-k]. */
 int dsdhip_pe_backward(const float *d_p, int B, size_t N, int ntaps, const double *d_gu,
                        double *d_gw, void *d_ws, size_t ws_bytes, void *stream);
+
+/* ======================================================================
+ * 9. Streaming chain for one multichannel stream: FIR upsampling, optional
+ *    UPOLS convolution, DAS, per chunk on one HIP stream.  Replaces the
+ *    per-channel engine_process_block chain (src/engine.c:478-729: FIR with
+ *    gain -> conv_process -> SDM) with DAS as the SDM, as the GPU plugin
+ *    path runs it (src/dsp_plugin.c:1941-2761).  The engine's soft clip
+ *    (src/engine.c:704-716) is not applied.
+ * ====================================================================== */
+typedef struct dsdhip_stream dsdhip_stream_t;
+/* ir == NULL: no convolution.  Otherwise ir is host f64 [nch][ir_stride]
+ * and chunk_frames * 2^stages must be a multiple of 8*P.  The DAS runs
+ * all segments from the persistent contexts (seg0_first = 0) with segments
+ * of seg_len samples. */
+dsdhip_stream_t *dsdhip_stream_create(unsigned fs_in, unsigned fs_out, double gain, int nch,
+                                      size_t chunk_frames, const double *ir, int64_t ir_len,
+                                      int64_t ir_stride, int P, int ntf_index, int depth, int cands,
+                                      int lat, double state_limit, size_t seg_len, int reposition);
+void dsdhip_stream_destroy(dsdhip_stream_t *s);
+int dsdhip_stream_set_stream(dsdhip_stream_t *s, void *hip_stream);
+/* replaces engine_channel_reset (src/engine.c:732) for every channel */
+int dsdhip_stream_reset(dsdhip_stream_t *s);
+int dsdhip_stream_ratio(const dsdhip_stream_t *s);
+/* One chunk: d_pcm float32 [nch][pcm_stride], frames <= chunk_frames;
+ * d_bits nch rows of out_words packed words (bit = 1 for +1.0). */
+int dsdhip_stream_process_dev(dsdhip_stream_t *s, const float *d_pcm, int64_t pcm_stride,
+                              size_t frames, unsigned long long *d_bits, int64_t out_words,
+                              size_t *out_count);
+/* SDM input of the last chunk into dst [nch][dst_stride] doubles (host or
+ * device); returns the samples per channel or -1.  Synchronises. */
+int64_t dsdhip_stream_copy_signal(dsdhip_stream_t *s, double *dst, int64_t dst_stride);
 
 /* Library version string. */
 const char *dsdhip_version(void);
