@@ -67,19 +67,28 @@ __device__ __forceinline__ unsigned shl_add_lt(unsigned acc, double k, double c)
 }
 
 /* Strict-order mask against the first NJ metrics of the LDS key array
- * (broadcast reads, unrolled by 16). */
+ * (broadcast reads).  Keys are read 16 at a time with 128-bit loads and
+ * consumed in load order, so one LDS round trip is exposed per 16 keys. */
 template <int NJ>
 __device__ __forceinline__ u64 lt_sweep(const double *key, double c)
 {
-    unsigned lo = 0, hi = 0;
-    /* descending j, shift-in: bit j of the word ends at position j % 32 */
-    if (NJ > 32) {
-#pragma unroll 16
-        for (int j = NJ - 1; j >= 32; j--) hi = shl_add_lt(hi, key[j], c);
+    const double2 *k2 = reinterpret_cast<const double2 *>(key);
+    unsigned w[2] = {0u, 0u};
+#pragma unroll
+    for (int b = NJ / 8 - 1; b >= 0; b--) {
+        double2 kv[4];
+#pragma unroll
+        for (int i = 3; i >= 0; i--) kv[i] = k2[b * 4 + i];
+        unsigned acc = w[b >> 2];
+#pragma unroll
+        for (int i = 3; i >= 0; i--) {
+            acc = shl_add_lt(acc, kv[i].y, c);
+            acc = shl_add_lt(acc, kv[i].x, c);
+        }
+        w[b >> 2] = acc;
+        __builtin_amdgcn_sched_barrier(0);
     }
-#pragma unroll 16
-    for (int j = (NJ < 32 ? NJ : 32) - 1; j >= 0; j--) lo = shl_add_lt(lo, key[j], c);
-    return ((u64)hi << 32) | lo;
+    return ((u64)w[1] << 32) | w[0];
 }
 
 /* LDS working set of one context: the per-slot records of both banks
@@ -286,13 +295,16 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
     u64 acc = 0;
     int nacc = 0;
     int64_t kept = 0, slow_steps = 0;
-    double xv = 0.0;
+    /* input in blocks of 64 samples, one lane each; the next block is
+     * loaded a block ahead */
+    double xv = lane < n_in ? in[lane] : 0.0, xn = 0.0;
 
     for (int64_t t = 0; t < n_in; t++) {
         const int tl = (int)(t & 63);
         if (tl == 0) {
-            const int64_t ti = t + lane;
-            xv = ti < n_in ? in[ti] : 0.0;
+            if (t > 0) xv = xn;
+            const int64_t ti = t + 64 + lane;
+            xn = ti < n_in ? in[ti] : 0.0;
         }
         const double x = rl_d(xv, tl);
 
@@ -394,7 +406,7 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
 
             /* fixed point of early skip + duplicate-path resolution */
             const u64 LTc = LT & CAND & lowerm;
-            u64 PASS = 0, DS = 0, KM = 0;
+            u64 DS = 0, KM = 0;
             int killt = 64, R = 64;
             bool pass = false, kil = false;
             u64 KILLERS = 0;
@@ -415,19 +427,15 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
                 kil = dup && !rlt;
                 const u64 nDS = ballot(rlt);
                 KILLERS = ballot(kil);
-                int nkt = 64;
-                u64 nK = 0;
-                for (u64 kk = KILLERS; kk; kk &= kk - 1) {
-                    const int s_ = ff1(kk);
-                    const int rs = rl_i(R, s_);
-                    if (!((nK >> rs) & 1)) {
-                        nK |= 1ull << rs;
-                        if (lane == rs) nkt = s_;
-                    }
-                }
-                const bool same = nPASS == PASS && nDS == DS && nK == KM &&
-                                  ballot(nkt != killt) == 0;
-                PASS = nPASS;
+                /* every killer of a registrant is in the registrant's path
+                 * group (R is the group's first passing lane), so a killed
+                 * registrant and its first killer follow from its own group */
+                const u64 gk = G & KILLERS;
+                const bool tgt = pass && R == lane && gk != 0;
+                const u64 nK = ballot(tgt);
+                const int nkt = tgt ? ff1(gk) : 64;
+                /* the iteration map depends on (DS, KM, killt) only */
+                const bool same = nDS == DS && nK == KM && ballot(nkt != killt) == 0;
                 DS = nDS;
                 KM = nK;
                 killt = nkt;
