@@ -25,6 +25,7 @@
 #include "tr_types.h"
 
 typedef unsigned long long u64;
+#define GPTR(T) T __attribute__((address_space(1))) *
 
 namespace {
 
@@ -48,6 +49,15 @@ __device__ __forceinline__ int mbcnt(u64 m)
                                           __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
+/* A 64-bit zero materialised where it is used (the register allocator
+ * otherwise keeps one zero pair alive across the step and spills it). */
+__device__ __forceinline__ u64 vzero64()
+{
+    unsigned lo, hi;
+    __asm__ volatile("v_mov_b32 %0, 0\n\tv_mov_b32 %1, 0" : "=v"(lo), "=v"(hi));
+    return ((u64)hi << 32) | lo;
+}
+
 /* The workgroup is one wave: its LDS instructions execute in issue order,
  * so cross-lane exchange through LDS only needs the compiler not to move
  * memory operations across this point. */
@@ -64,6 +74,46 @@ __device__ __forceinline__ unsigned shl_add_lt(unsigned acc, double k, double c)
             : "v"(k), "v"(c), "v"(acc)
             : "vcc");
     return r;
+}
+
+/* acc * 2 + (k < c) for unsigned 32-bit keys */
+__device__ __forceinline__ unsigned shl_add_ltu(unsigned acc, unsigned k, unsigned c)
+{
+    unsigned r;
+    __asm__("v_cmp_lt_u32_e32 vcc, %1, %2\n\t"
+            "v_addc_co_u32_e32 %0, vcc, %3, %3, vcc"
+            : "=v"(r)
+            : "v"(k), "v"(c), "v"(acc)
+            : "vcc");
+    return r;
+}
+
+/* Strict-order mask on the metrics' high words: for non-negative doubles
+ * the order of the bit patterns is the order of the values, so where all
+ * high words differ this is the exact strict order.  16 keys per batch
+ * (four 128-bit loads). */
+template <int NJ>
+__device__ __forceinline__ u64 lt_sweep_hi(const uint32_t *key, unsigned c)
+{
+    const uint4 *k4 = reinterpret_cast<const uint4 *>(key);
+    unsigned w[2] = {0u, 0u};
+#pragma unroll
+    for (int b = NJ / 16 - 1; b >= 0; b--) {
+        uint4 kv[4];
+#pragma unroll
+        for (int i = 3; i >= 0; i--) kv[i] = k4[b * 4 + i];
+        unsigned acc = w[b >> 1];
+#pragma unroll
+        for (int i = 3; i >= 0; i--) {
+            acc = shl_add_ltu(acc, kv[i].w, c);
+            acc = shl_add_ltu(acc, kv[i].z, c);
+            acc = shl_add_ltu(acc, kv[i].y, c);
+            acc = shl_add_ltu(acc, kv[i].x, c);
+        }
+        w[b >> 1] = acc;
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    return ((u64)w[1] << 32) | w[0];
 }
 
 /* Strict-order mask against the first NJ metrics of the LDS key array
@@ -101,6 +151,7 @@ template <int P, int HW>
 struct TrLds {
     double ncost[2][64];
     uint32_t npath[2][64];
+    uint32_t khi[64];              /* high words of this step's child metrics */
     u64 gtab[256 + 64];            /* + one scratch entry per lane */
     u64 btab[128 + 64];
     int8_t link[2][64];
@@ -290,21 +341,25 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
 
     const double a0 = prm.a[0];
     const u64 lowerm = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const double *in = jb->in;
+    /* global address space: these accesses count against vmcnt only, so the
+     * prefetched block stays in flight across the step's LDS waits */
+    const GPTR(double) in = (const GPTR(double))jb->in;
+    GPTR(u64) out = (GPTR(u64))jb->out;
     const int64_t n_in = jb->n_in, keep_from = jb->keep_from;
     u64 acc = 0;
     int nacc = 0;
     int64_t kept = 0, slow_steps = 0;
     /* input in blocks of 64 samples, one lane each; the next block is
      * loaded a block ahead */
-    double xv = lane < n_in ? in[lane] : 0.0, xn = 0.0;
+    double xv = 0.0, xn = 0.0;
+    if (n_in > 0) xv = in[lane < n_in ? lane : n_in - 1];
 
     for (int64_t t = 0; t < n_in; t++) {
         const int tl = (int)(t & 63);
         if (tl == 0) {
             if (t > 0) xv = xn;
             const int64_t ti = t + 64 + lane;
-            xn = ti < n_in ? in[ti] : 0.0;
+            xn = in[ti < n_in ? ti : n_in - 1];    /* lanes past the end are never read */
         }
         const double x = rl_d(xv, tl);
 
@@ -357,25 +412,47 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
         const int cnext = pnext;
         if (ballot(live && !__builtin_isfinite(cost))) err = 1;
 
+        const unsigned chi = (unsigned)((u64)bits(cost) >> 32);
         if (live) {
             s.npath[nb][lane] = path;
             s.ncost[nb][lane] = cost;      /* also the sweep's keys: entries >= m are never counted */
+            s.khi[lane] = chi;
         }
         wsync();
 
         /* ---- 2. prune ---- */
         const u64 ALL = (m == 64) ? ~0ull : ((1ull << m) - 1ull);
         /* LT bit j: child j has a strictly smaller metric (bits >= m are
-         * don't-care; every use masks them off) */
-        const double *keys = s.ncost[nb];
-        const u64 LT = (m <= 16) ? lt_sweep<16>(keys, cost)
-                     : (m <= 32) ? lt_sweep<32>(keys, cost) : lt_sweep<64>(keys, cost);
-
-        /* equal metrics: strict ranks must be distinct */
-        const int rall = __popcll(LT & ALL);
-        if (live) s.rank[rall] = (uint8_t)lane;
-        wsync();
-        bool slow = !groups_ok || (ballot(live && s.rank[rall] != lane) != 0);
+         * don't-care; every use masks them off).  First on the high words
+         * (metrics are >= 0; a set sign bit takes the full compare); equal
+         * high words give equal strict ranks, and only then the full 64-bit
+         * sweep runs. */
+        u64 LT;
+        int rall;
+        bool tie;
+        {
+            const bool neg = ballot(live && (chi >> 31)) != 0;
+            if (!neg) {
+                LT = (m <= 16) ? lt_sweep_hi<16>(s.khi, chi)
+                   : (m <= 32) ? lt_sweep_hi<32>(s.khi, chi) : lt_sweep_hi<64>(s.khi, chi);
+                rall = __popcll(LT & ALL);
+                if (live) s.rank[rall] = (uint8_t)lane;
+                wsync();
+                tie = ballot(live && s.rank[rall] != lane) != 0;
+                wsync();
+            }
+            if (neg || tie) {
+                const double *keys = s.ncost[nb];
+                LT = (m <= 16) ? lt_sweep<16>(keys, cost)
+                   : (m <= 32) ? lt_sweep<32>(keys, cost) : lt_sweep<64>(keys, cost);
+                /* equal metrics: strict ranks must be distinct */
+                rall = __popcll(LT & ALL);
+                if (live) s.rank[rall] = (uint8_t)lane;
+                wsync();
+                tie = ballot(live && s.rank[rall] != lane) != 0;
+            }
+        }
+        bool slow = !groups_ok || tie;
 
         const u64 N1 = ballot(live && cnext);
         int nsurv = 0, myrank = 255;
@@ -402,7 +479,7 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
             wsync();
             const u64 G = s.gtab[gi] & (cand ? ~0ull : 0ull);
             wsync();
-            s.gtab[gi] = 0ull;
+            s.gtab[gi] = vzero64();
 
             /* fixed point of early skip + duplicate-path resolution */
             const u64 LTc = LT & CAND & lowerm;
@@ -464,7 +541,7 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
                 const u64 bm = s.btab[bi] & lowerm;
                 if (reg) s.link[nb][lane] = bm ? (int8_t)(63 - __builtin_clzll(bm)) : (int8_t)-1;
                 wsync();
-                s.btab[bi] = 0ull;
+                s.btab[bi] = vzero64();
                 const u64 F = CAND & ~DS & ~KM;
                 const int rk = __popcll(LT & F);
                 myrank = (((F >> lane) & 1) && rk < K) ? rk : 255;
@@ -528,7 +605,7 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
         } else if (t >= keep_from) {
             acc |= (u64)outbit << nacc;
             if (++nacc == 64) {
-                if (lane == 0) jb->out[kept >> 6] = acc;
+                if (lane == 0) out[kept >> 6] = acc;
                 kept += 64;
                 acc = 0;
                 nacc = 0;
@@ -536,7 +613,7 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
         }
     }
     if (nacc) {
-        if (lane == 0) jb->out[kept >> 6] = acc;
+        if (lane == 0) out[kept >> 6] = acc;
         kept += nacc;
     }
 
