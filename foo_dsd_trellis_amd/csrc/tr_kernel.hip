@@ -49,6 +49,25 @@ __device__ __forceinline__ int mbcnt(u64 m)
                                           __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
+/* Exclusive prefix OR over the lanes of a 64-bit value whose set bits are
+ * distinct across lanes (inclusive scan by DPP row shifts and row
+ * broadcasts, then the lane's own bits removed). */
+__device__ __forceinline__ unsigned scan_or32(unsigned x)
+{
+    x |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true);   /* row_shr:1 */
+    x |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true);   /* row_shr:2 */
+    x |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true);   /* row_shr:4 */
+    x |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true);   /* row_shr:8 */
+    x |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);  /* row_bcast:15 */
+    x |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);  /* row_bcast:31 */
+    return x;
+}
+__device__ __forceinline__ u64 wave_excl_or(u64 v)
+{
+    const u64 inc = ((u64)scan_or32((unsigned)(v >> 32)) << 32) | scan_or32((unsigned)v);
+    return inc & ~v;
+}
+
 /* A 64-bit zero materialised where it is used (the register allocator
  * otherwise keeps one zero pair alive across the step and spills it). */
 __device__ __forceinline__ u64 vzero64()
@@ -484,16 +503,16 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
             /* fixed point of early skip + duplicate-path resolution */
             const u64 LTc = LT & CAND & lowerm;
             u64 DS = 0, KM = 0;
-            int killt = 64, R = 64;
+            /* ktgt: in the first killer of a registrant, that registrant */
+            int ktgt = 64, R = 64;
             bool pass = false, kil = false;
             u64 KILLERS = 0;
             int it;
             for (it = 0; it < 64; it++) {
+                /* registrants already removed when this lane is reached:
+                 * those whose first killer comes before it */
                 u64 kb = 0;
-                for (u64 km = KM; km; km &= km - 1) {
-                    const int dd = ff1(km);
-                    if (lane > rl_i(killt, dd)) kb |= 1ull << dd;
-                }
+                if (KM) kb = wave_excl_or(ktgt < 64 ? (1ull << ktgt) : 0ull);
                 const int cnt = __popcll(LTc & ~DS & ~kb);
                 pass = cand && cnt < K;
                 const u64 nPASS = ballot(pass);
@@ -505,17 +524,16 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
                 const u64 nDS = ballot(rlt);
                 KILLERS = ballot(kil);
                 /* every killer of a registrant is in the registrant's path
-                 * group (R is the group's first passing lane), so a killed
-                 * registrant and its first killer follow from its own group */
+                 * group (R is the group's first passing lane): the group's
+                 * first killer removes it */
                 const u64 gk = G & KILLERS;
-                const bool tgt = pass && R == lane && gk != 0;
-                const u64 nK = ballot(tgt);
-                const int nkt = tgt ? ff1(gk) : 64;
-                /* the iteration map depends on (DS, KM, killt) only */
-                const bool same = nDS == DS && nK == KM && ballot(nkt != killt) == 0;
+                const u64 nK = ballot(pass && R == lane && gk != 0);
+                const int nkg = (kil && lane == ff1(gk)) ? R : 64;
+                /* the iteration map depends on (DS, KM, ktgt) only */
+                const bool same = nDS == DS && nK == KM && ballot(nkg != ktgt) == 0;
                 DS = nDS;
                 KM = nK;
-                killt = nkt;
+                ktgt = nkg;
                 if (same) break;
             }
             if (it >= 64) slow = true;
