@@ -529,8 +529,12 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
                 const u64 gk = G & KILLERS;
                 const u64 nK = ballot(pass && R == lane && gk != 0);
                 const int nkg = (kil && lane == ff1(gk)) ? R : 64;
-                /* the iteration map depends on (DS, KM, ktgt) only */
-                const bool same = nDS == DS && nK == KM && ballot(nkg != ktgt) == 0;
+                /* the iteration map depends on (DS, KM, ktgt) only.  Removals
+                 * only lower the counts, so every pass set contains the first
+                 * one (computed with no removals): when that first set is all
+                 * of CAND it never changes and this state is the fixed point */
+                const bool same = (it == 0 && nPASS == CAND) ||
+                                  (nDS == DS && nK == KM && ballot(nkg != ktgt) == 0);
                 DS = nDS;
                 KM = nK;
                 ktgt = nkg;
