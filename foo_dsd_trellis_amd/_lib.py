@@ -43,6 +43,9 @@ _SIGS = {
     "dsdhip_trellis_run_jobs": ([_vp, C.c_int, _ip, _ip, C.POINTER(_vp), _i64p, _i64p,
                                  C.POINTER(_vp), _dp, _i64p, _i64p, _ip], C.c_int),
     "dsdhip_trellis_force_literal": ([_vp, C.c_int], C.c_int),
+    "dsdhip_trellis_fma_o8": ([_vp, C.c_int], C.c_int),
+    "dsdhip_das_fma_o8": ([_vp, C.c_int], C.c_int),
+    "gpu_hip_trellis_fma_o8": ([_vp, C.c_int], C.c_int),
     "dsdhip_fir_plan": ([C.c_uint, C.c_uint, _ip], C.c_int),
     "dsdhip_halfband_taps": ([_dp, C.c_int], None),
     "dsdhip_fir_create": ([C.c_uint, C.c_uint, C.c_double, C.c_int], _vp),

@@ -27,6 +27,7 @@ struct gpu_hip_context {
     int das_nch;
     size_t das_seg_len;
     int das_seg0_first, das_reposition;
+    int fma_o8;
     /* uniform-taps FIR setup (gpu_hip_fir_setup) */
     double taps[127];
     int ntaps, nstages;
@@ -158,6 +159,7 @@ int gpu_hip_trellis_setup(void *ctx, int num_cands, int order, int trellis_lat, 
     c->tr = dsdhip_trellis_create_coeffs(ntf_a, ntf_g, order, trellis_depth, num_cands, trellis_lat,
                                          state_limit, 1);
     if (!c->tr) return -1;
+    dsdhip_trellis_fma_o8(c->tr, c->fma_o8);
     memset(c->a, 0, sizeof(c->a));
     memset(c->g, 0, sizeof(c->g));
     memcpy(c->a, ntf_a, sizeof(double) * (size_t)order);
@@ -193,6 +195,16 @@ int gpu_hip_das_config(void *ctx, size_t seg_len, int seg0_first, int reposition
     return 0;
 }
 
+int gpu_hip_trellis_fma_o8(void *ctx, int on)
+{
+    gpu_hip_context *c = (gpu_hip_context *)ctx;
+    if (!c) return -1;
+    c->fma_o8 = on ? 1 : 0;
+    if (c->tr) dsdhip_trellis_fma_o8(c->tr, c->fma_o8);
+    if (c->das) dsdhip_das_fma_o8(c->das, c->fma_o8);
+    return 0;
+}
+
 int gpu_hip_trellis_das(void *ctx, const double *in, float *out, size_t count, int num_channels)
 {
     gpu_hip_context *c = (gpu_hip_context *)ctx;
@@ -205,6 +217,7 @@ int gpu_hip_trellis_das(void *ctx, const double *in, float *out, size_t count, i
                                           c->das_reposition);
         c->das_nch = num_channels;
         if (!c->das) return -1;
+        dsdhip_das_fma_o8(c->das, c->fma_o8);
     }
     size_t n = 0;
     return dsdhip_das_run(c->das, in, count, out, &n);

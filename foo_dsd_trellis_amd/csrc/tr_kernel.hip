@@ -307,7 +307,7 @@ __device__ int literal_prune(TrLds<P, HW> &s, int nb, int m, int K,
 /* One job on one wave; s is the wave's own LDS region.  Nothing here
  * synchronises beyond the wave, so several waves of a workgroup may run
  * different jobs (and different numbers of them). */
-template <int P, int HW>
+template <int P, int HW, bool F8 = false>
 __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr_ctx *ctxs,
                                             TrLds<P, HW> &s, const int lane)
 {
@@ -415,6 +415,24 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
         }
         d[P - 1] = sp[P - 1] + sp[P - 2];
         v = v + prm.a[P - 1] * d[P - 1];
+        if constexpr (F8) {
+            /* the reference's AVX2 build runs candidates four at a time while
+             * four remain, with a fused order-8 filter (src/trellis.c:395-421,
+             * 904-927): every d[i] fuses its g term, v accumulates by FMA.
+             * Computed beside the scalar form and selected per lane. */
+            const bool f = k < (n & ~3);
+            double df[P];
+            df[0] = __fma_rn(-prm.g[0], sp[1], sp[0] + x);
+#pragma unroll
+            for (int q = 1; q < P - 1; q++) df[q] = __fma_rn(-prm.g[q], sp[q + 1], sp[q] + sp[q - 1]);
+            df[P - 1] = d[P - 1];
+            double vf = __fma_rn(prm.a[0], df[0], x);
+#pragma unroll
+            for (int q = 1; q < P; q++) vf = __fma_rn(prm.a[q], df[q], vf);
+#pragma unroll
+            for (int q = 0; q < P - 1; q++) d[q] = f ? df[q] : d[q];
+            v = f ? vf : v;
+        }
         d[0] = d[0] + (br ? -1.0 : 1.0);
         if (prm.state_limit > 0.0) {
             /* src/trellis.c:179-187, as selects (no per-lane branches) */
@@ -695,6 +713,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TrWaves<HW>:
     tr_job_body<P, HW>(prm, jb, ctxs, s, threadIdx.x);
 }
 
+/* The same with the order-8 FMA filter of the reference's AVX2 build. */
+template <int HW>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TrWaves<HW>::value, 8))) void tr_run_jobs_f8(tr_params prm, tr_job *jobs, tr_ctx *ctxs)
+{
+    __shared__ TrLds<8, HW> s;
+    tr_job *jb = &jobs[blockIdx.x];
+    if (jb->skip) return;
+    tr_job_body<8, HW, true>(prm, jb, ctxs, s, threadIdx.x);
+}
+
 /* Persistent form: WPB waves per workgroup, each taking the next job slot
  * from a device counter until `njobs` slots are handed out; slots marked
  * `skip` are passed over.  Every wave leaves once the counter is past the
@@ -727,6 +755,12 @@ TR_INST_P(5)
 TR_INST_P(6)
 TR_INST_P(7)
 TR_INST_P(8)
+template __global__ void tr_run_jobs_f8<1>(tr_params, tr_job *, tr_ctx *);
+template __global__ void tr_run_jobs_f8<2>(tr_params, tr_job *, tr_ctx *);
+template __global__ void tr_run_jobs_f8<4>(tr_params, tr_job *, tr_ctx *);
+template __global__ void tr_run_jobs_f8<8>(tr_params, tr_job *, tr_ctx *);
+template __global__ void tr_run_jobs_f8<16>(tr_params, tr_job *, tr_ctx *);
+template __global__ void tr_run_jobs_f8<32>(tr_params, tr_job *, tr_ctx *);
 
 /* Host-side launcher table (P in 4..8, HW in {1,2,4,8,16,32}). */
 extern "C" int tr_launch_jobs(const tr_params *prm, tr_job *d_jobs, int njobs,
@@ -739,6 +773,14 @@ extern "C" int tr_launch_jobs(const tr_params *prm, tr_job *d_jobs, int njobs,
     while (HW < hw_need) HW *= 2;
     if (P < 4 || P > 8 || HW > 32) return -1;
     dim3 grid((unsigned)njobs), block(64);
+    if (prm->fma_o8 && P == 8) {
+#define TF_CASE(HH) \
+        if (HW == HH) { \
+            hipLaunchKernelGGL((tr_run_jobs_f8<HH>), grid, block, 0, stream, *prm, d_jobs, d_ctxs); \
+            return hipGetLastError() == hipSuccess ? 0 : -1; }
+        TF_CASE(1) TF_CASE(2) TF_CASE(4) TF_CASE(8) TF_CASE(16) TF_CASE(32)
+        return -1;
+    }
 #define TR_CASE(PP, HH) \
     if (P == PP && HW == HH) { \
         hipLaunchKernelGGL((tr_run_jobs<PP, HH>), grid, block, 0, stream, *prm, d_jobs, d_ctxs); \
@@ -759,6 +801,7 @@ extern "C" int tr_launch_queue(const tr_params *prm, tr_job *d_jobs, int njobs, 
     while (HW < hw_need) HW *= 2;
     if (P < 4 || P > 8 || HW > 32 || nblocks < 1) return -1;
     if (wpb != 1 && wpb != 2 && wpb != 4) return -1;
+    if (prm->fma_o8 && P == 8) return -1;       /* the FMA filter runs in tr_run_jobs only */
     dim3 grid((unsigned)nblocks), block(64 * wpb);
 #define TQ_CASE(PP, HH, WW) \
     if (P == PP && HW == HH && wpb == WW) { \

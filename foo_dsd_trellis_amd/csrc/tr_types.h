@@ -47,6 +47,7 @@ typedef struct tr_params {
     int32_t P, K, L, B;
     uint32_t mask;
     int32_t force_slow;     /* validation: decide every step with the literal prune */
+    int32_t fma_o8;         /* order 8: the 4-candidate FMA filter of the AVX2 build */
 } tr_params;
 
 /* One unit of trellis work: run `n_in` inputs through a context.

@@ -39,6 +39,10 @@ class Das:
     def force_literal(self, on=True):
         check(lib().dsdhip_das_force_literal(self.h, 1 if on else 0), "force_literal")
 
+    def fma_o8(self, on=True):
+        """Order 8: the reference AVX2 build's 4-candidate FMA filter."""
+        check(lib().dsdhip_das_fma_o8(self.h, 1 if on else 0), "fma_o8")
+
     def nseg(self, F):
         return lib().dsdhip_das_plan(self.h, F, None, None)
 

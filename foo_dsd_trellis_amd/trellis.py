@@ -61,6 +61,10 @@ class SdmPool:
     def force_literal(self, on=True):
         check(lib().dsdhip_trellis_force_literal(self.h, 1 if on else 0), "force_literal")
 
+    def fma_o8(self, on=True):
+        """Order 8: the reference AVX2 build's 4-candidate FMA filter."""
+        check(lib().dsdhip_trellis_fma_o8(self.h, 1 if on else 0), "fma_o8")
+
     def reset(self, i):
         check(lib().dsdhip_trellis_reset(self.h, i), "reset")
 

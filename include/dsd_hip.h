@@ -80,6 +80,11 @@ int gpu_hip_trellis(void *ctx, const double *in, float *out, size_t count);
  * num_channels x count samples (channel-major), all segments in one launch,
  * same semantics as the reference's CPU DAS (src/dsp_plugin.c:1314-2761). */
 int gpu_hip_trellis_das(void *ctx, const double *in, float *out, size_t count, int num_channels);
+/* Bit parity with the shipped Windows build: order-8 filters computed by
+ * the reference's AVX2 4-candidate FMA path (src/trellis.c:348-533, chosen
+ * at :904-927 for the first 4*floor(n/4) candidates of every step) instead
+ * of the scalar order.  Extension of the seam; off by default. */
+int gpu_hip_trellis_fma_o8(void *ctx, int on);
 /* DAS plan parameters for gpu_hip_trellis_das (segment length in samples,
  * 0 = reference auto count; seed mode 1 = seg0-first, 0 = all segments
  * seeded from the persistent state; reposition quiet boundaries). */
@@ -139,6 +144,9 @@ int dsdhip_trellis_run_jobs(dsdhip_trellis_t *t, int njobs, const int *ctx, cons
                             int *err_out);
 /* Validation switch: decide every step with the literal sequential prune. */
 int dsdhip_trellis_force_literal(dsdhip_trellis_t *t, int on);
+/* Order 8: the AVX2 build's 4-candidate FMA filter (src/trellis.c:348-533,
+ * 904-927) instead of the scalar one (see gpu_hip_trellis_fma_o8). */
+int dsdhip_trellis_fma_o8(dsdhip_trellis_t *t, int on);
 /* HIP-event timing of trellis launches on the pool's stream (no reference
  * counterpart; the reference times phases with QueryPerformanceCounter,
  * src/dsp_plugin.c:90-94). */
@@ -191,6 +199,7 @@ dsdhip_das_t *dsdhip_das_create_coeffs(const double *a, const double *g, int ord
 void dsdhip_das_destroy(dsdhip_das_t *d);
 int dsdhip_das_set_stream(dsdhip_das_t *d, void *hip_stream);
 int dsdhip_das_force_literal(dsdhip_das_t *d, int on);
+int dsdhip_das_fma_o8(dsdhip_das_t *d, int on);
 /* Segment count (and nominal starts/sizes if given) for F samples. */
 int dsdhip_das_plan(const dsdhip_das_t *d, size_t F, size_t *start, size_t *size);
 /* One chunk: d_in nch x F doubles (row stride in_stride); d_out nch rows

@@ -77,6 +77,7 @@ typedef struct {
     uint32_t mask;
     unsigned num_cands, pos, pending, draining, idx;
     double state_limit;
+    int fma_o8;                /* 4-wide FMA filter of the shipped AVX2 build */
     uint64_t conv_fail, cands_collapse, next_filter_drops, total_children;
 } or_ctx;
 
@@ -94,6 +95,11 @@ typedef struct {
 typedef void (*or_prune_observer_fn)(const or_prune_obs *);
 extern or_prune_observer_fn or_prune_observer;
 
+/* Contexts initialised while this is set use the order-8 4-wide FMA filter
+ * of the reference's AVX2 build (src/trellis.c:348-533, 904-928) for the
+ * first 4*floor(n/4) candidates of every step; parity pinned against the
+ * reference's own trellis.c compiled with that path selected. */
+extern int or_fma_o8_default;
 int or_sdm_init(or_ctx *c, const or_ntf_t *f, int depth, int cands, int lat);
 void or_sdm_reset(or_ctx *c);
 /* One trellis step; returns the decided bit (1 => +1.0). */

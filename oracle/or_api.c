@@ -213,3 +213,6 @@ void ora_das_stream_free(void *hv)
     free(h->pc);
     free(h);
 }
+
+/* Select the order-8 FMA filter for contexts created from now on. */
+void ora_set_fma_o8(int on) { or_fma_o8_default = on ? 1 : 0; }

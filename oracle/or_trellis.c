@@ -76,6 +76,23 @@ static double filt(const double *s, double *d, const or_ntf_t *f, double x)
     return v;
 }
 
+/* The order-8 filter of the reference's 4-candidate AVX2 step
+ * (src/trellis.c:395-421): each d[i] fuses its g term, v accumulates by
+ * FMA in ascending i. */
+static double filt_fma8(const double *s, double *d, const or_ntf_t *f, double x)
+{
+    const double *a = f->a, *g = f->g;
+    int i;
+    d[0] = fma(-g[0], s[1], s[0] + x);
+    for (i = 1; i < 7; i++) d[i] = fma(-g[i], s[i + 1], s[i] + s[i - 1]);
+    d[7] = s[7] + s[6];
+    double v = fma(a[0], d[0], x);
+    for (i = 1; i < 8; i++) v = fma(a[i], d[i], v);
+    return v;
+}
+
+int or_fma_o8_default = 0;
+
 static void clamp_state(double *s, int P, double lim)
 {
     for (int i = 0; i < P; i++) {
@@ -91,6 +108,7 @@ int or_sdm_init(or_ctx *c, const or_ntf_t *f, int depth, int cands, int lat)
     if (depth > 32 || cands > OR_MAXK || lat > OR_MAXL) return -1;
     if (depth < 1 || cands < 1 || lat < 1) return -1;
     c->f = f;
+    c->fma_o8 = or_fma_o8_default;
     c->K = (unsigned)cands;
     c->L = (unsigned)lat;
     c->mask = (uint32_t)(((uint64_t)1 << depth) - 1);
@@ -238,10 +256,13 @@ int or_sdm_sample(or_ctx *c, double x)
     if (next_pos == c->L) next_pos = 0;
     unsigned hw = (c->L + 63) / 64;
 
+    /* the AVX2 build takes candidates four at a time while four remain
+     * (src/trellis.c:904-927) */
+    const unsigned nfma = (c->fma_o8 && P == 8) ? (c->num_cands & ~3u) : 0;
     for (unsigned i = 0; i < c->num_cands; i++) {
         or_node *p = c->act[cur][i];
         or_node *ch = &c->node[nxt][2 * i];
-        double v = filt(p->st, ch[0].st, f, x);
+        double v = i < nfma ? filt_fma8(p->st, ch[0].st, f, x) : filt(p->st, ch[0].st, f, x);
         for (int k = 0; k < P; k++) ch[1].st[k] = ch[0].st[k];
         ch[0].st[0] += 1.0;
         ch[1].st[0] -= 1.0;

@@ -36,6 +36,15 @@ def reference():
 
 
 @pytest.fixture(scope="session")
+def reference_fma():
+    import oracle_lib
+    r = oracle_lib.reference_fma()
+    if r is None:
+        pytest.skip("reference sources not built here (oracle/_ref)")
+    return r
+
+
+@pytest.fixture(scope="session")
 def gpu():
     if not gpu_available():
         pytest.fail("GPU test run without a usable GPU / libdsdhip.so")

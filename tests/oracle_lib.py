@@ -13,6 +13,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libdsdref.so")
+REF_FMA_SO = os.path.join(ROOT, "oracle", "_ref", "libdsdref_fma.so")
 
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
@@ -145,6 +146,29 @@ def reference():
     if "ref" not in _cache:
         _cache["ref"] = TrellisLib(REF_SO, "ref") if os.path.exists(REF_SO) else None
     return _cache["ref"]
+
+
+def reference_fma():
+    """The reference compiled with its shipped AVX2/FMA path selected, or None."""
+    if "ref_fma" not in _cache:
+        _cache["ref_fma"] = TrellisLib(REF_FMA_SO, "ref") if os.path.exists(REF_FMA_SO) else None
+    return _cache["ref_fma"]
+
+
+class fma_o8:
+    """Context manager: oracle contexts created inside use the order-8 FMA
+    filter of the reference's AVX2 build."""
+
+    def __init__(self, lib_):
+        self.L = lib_.lib
+        self.L.ora_set_fma_o8.argtypes = [C.c_int]
+
+    def __enter__(self):
+        self.L.ora_set_fma_o8(1)
+        return self
+
+    def __exit__(self, *a):
+        self.L.ora_set_fma_o8(0)
 
 
 def pack_bits(pm1):
