@@ -360,6 +360,77 @@ int dsdhip_stream_process_dev(dsdhip_stream_t *s, const float *d_pcm, int64_t pc
  * device); returns the samples per channel or -1.  Synchronises. */
 int64_t dsdhip_stream_copy_signal(dsdhip_stream_t *s, double *dst, int64_t dst_stride);
 
+/* ======================================================================
+ * 10. DSD wire formats: DoP and raw bit bytes (include/dop.h:12-48,
+ *     src/dop.c).  Device DSD streams are packed bits: sample i of a
+ *     channel is bit i % 64 of 64-bit word i / 64 of that channel's row,
+ *     1 = +1.0.  Bits past a stream's count are 0.
+ * ====================================================================== */
+/* replaces dop_detect_interleaved (include/dop.h:21; channels == 1 is
+ * dop_detect, :16): host frames, returns 1 if DoP markers are present. */
+int dsdhip_dop_detect(const float *pcm24, size_t frames, int channels);
+/* replaces the per-channel de-interleave + dop_unpack of the plugin's input
+ * stage (src/dsp_plugin.c:1254-1276, include/dop.h:26): interleaved DoP
+ * frames [frames][nch] -> packed bits [nch][words_stride], 16 per frame. */
+int dsdhip_dop_unpack_dev(const float *d_pcm24, size_t frames, int nch, unsigned long long *d_bits,
+                          int64_t words_stride, void *stream);
+/* replaces dop_pack (include/dop.h:31) per channel, interleaved output
+ * [bit_count/16][nch] floats; markers alternate from 0x05 at frame 0. */
+int dsdhip_dop_pack_dev(const unsigned long long *d_bits, int64_t words_stride, int nch, size_t bit_count,
+                        float *d_pcm24, void *stream);
+/* replaces dop_pack_i24 + interleave of the plugin's output stage
+ * (include/dop.h:40, src/dsp_plugin.c:2869-2945): [bit_count/16][nch][3]
+ * bytes (dsd_lo, dsd_hi, marker); returns the next marker phase or -1. */
+int dsdhip_dop_pack_i24_dev(const unsigned long long *d_bits, int64_t words_stride, int nch, size_t bit_count,
+                            int marker_phase, unsigned char *d_out, void *stream);
+/* replace bits_unpack / bits_pack (include/dop.h:44-48): raw MSB-first
+ * bytes [nch][stride] <-> packed bits. */
+int dsdhip_bytes_unpack_dev(const unsigned char *d_src, int64_t src_stride, int nch, size_t n_bits,
+                            unsigned long long *d_bits, int64_t words_stride, void *stream);
+int dsdhip_bytes_pack_dev(const unsigned long long *d_bits, int64_t words_stride, int nch, size_t n_bits,
+                          unsigned char *d_dst, int64_t dst_stride, void *stream);
+/* packed bits -> +-1.0f [nch][out_stride] (the plugin's ch_in form). */
+int dsdhip_bits_to_f32_dev(const unsigned long long *d_bits, int64_t words_stride, int nch, size_t n,
+                           float *d_out, int64_t out_stride, void *stream);
+
+/* packed bits <- +-1.0f samples (x >= 0 -> 1, as src/engine.c:515 reads
+ * DSD input). */
+int dsdhip_f32_to_bits_dev(const float *d_in, int64_t in_stride, int nch, size_t n, unsigned long long *d_bits,
+                           int64_t words_stride, void *stream);
+
+/* ======================================================================
+ * 11. Engine pre-SDM conditioning (src/engine.c:117-899): the boxcar of
+ *     a same-rate DSD stream (exact window sums of +-1), the 127-tap
+ *     lowpass alternative (src/fir.c:795-873), the boxcar pre-smooth of
+ *     PCM before rate conversion (a running double sum, sequential per
+ *     channel), the gain ramp and the tanh soft clip.  All FP64 in the
+ *     reference's operation order.  Histories persist between calls like
+ *     the engine channel's boxcar ring and lowpass delay line.
+ * ====================================================================== */
+typedef struct dsdhip_cond dsdhip_cond_t;
+/* fir_lowpass_init's design for a DSD rate (127 taps); returns 127. */
+int dsdhip_lowpass_taps(unsigned rate, double *h);
+/* nch channels, boxcar_taps (1..64; the engine's taps, src/engine.c:131-
+ * 147), lowpass_rate 0 = no lowpass, else its design rate. */
+dsdhip_cond_t *dsdhip_cond_create(int nch, int boxcar_taps, unsigned lowpass_rate);
+void dsdhip_cond_destroy(dsdhip_cond_t *c);
+/* replaces the boxcar / lowpass part of engine_channel_reset (:732-757) */
+int dsdhip_cond_reset(dsdhip_cond_t *c);
+int dsdhip_cond_set_stream(dsdhip_cond_t *c, void *hip_stream);
+/* same-rate DSD branch of engine_process_fir_gain (:810-846): packed bits
+ * -> f64, boxcar (or lowpass) times the ramp gs -> ge over the n samples */
+int dsdhip_cond_dsd_boxcar_dev(dsdhip_cond_t *c, const unsigned long long *d_bits, int64_t words_stride,
+                               size_t n, double gs, double ge, double *d_out, int64_t out_stride);
+int dsdhip_cond_dsd_lowpass_dev(dsdhip_cond_t *c, const unsigned long long *d_bits, int64_t words_stride,
+                                size_t n, double gs, double ge, double *d_out, int64_t out_stride);
+/* boxcar pre-smooth of PCM before the FIR chain (:629-642), no gain */
+int dsdhip_cond_pcm_boxcar_dev(dsdhip_cond_t *c, const float *d_in, int64_t in_stride, size_t n,
+                               double *d_out, int64_t out_stride);
+/* gain ramp in place: x[i] *= gs + (ge - gs) / (n - 1) * i (:646-652) */
+int dsdhip_gain_ramp_dev(double *d_x, int64_t stride, int nch, size_t n, double gs, double ge, void *stream);
+/* soft clip in place above +-0.95 with a tanh knee (:704-716) */
+int dsdhip_soft_clip_dev(double *d_x, int64_t stride, int nch, size_t n, void *stream);
+
 /* Library version string. */
 const char *dsdhip_version(void);
 

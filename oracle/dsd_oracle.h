@@ -175,4 +175,32 @@ void or_pe_backward(int T, const float *p, int B, size_t N, const double *gu, do
 #ifdef __cplusplus
 }
 #endif
+/* DSD wire formats (or_dop.c): src/dop.c restated; float +-1.0 streams
+ * as the reference's API uses them.  Pinned against the compiled dop.c
+ * and test/test_dop.c's known answers. */
+int or_dop_detect(const float *pcm, size_t frames, int channels);
+void or_dop_unpack(const float *pcm, float *bits, size_t frames);
+void or_dop_pack(const float *bits, float *pcm, size_t bit_count);
+int or_dop_pack_i24(const float *bits, uint8_t *out, size_t bit_count, int phase);
+void or_bits_unpack(const uint8_t *src, float *dst, size_t n_bits);
+void or_bits_pack(const float *src, uint8_t *dst, size_t n_bits);
+
+/* Engine pre-SDM conditioning (or_cond.c), one channel. */
+typedef struct {
+    int taps;                 /* boxcar taps */
+    double ring[64];
+    int pos;
+    double sum;
+    int lp_n;                 /* lowpass taps (0: none) */
+    double lp[127];
+    double dly[127];          /* lowpass delay line, dly[0] newest */
+} or_cond;
+int or_lowpass_taps(unsigned rate, double *h);
+void or_cond_init(or_cond *c, int taps, unsigned lowpass_rate);
+void or_cond_dsd_boxcar(or_cond *c, const float *in, size_t n, double gs, double ge, double *out);
+void or_cond_pcm_boxcar(or_cond *c, const float *in, size_t n, double *out);
+void or_cond_dsd_lowpass(or_cond *c, const float *in, size_t n, double gs, double ge, double *out);
+void or_gain_ramp(double *x, size_t n, double gs, double ge);
+void or_soft_clip(double *x, size_t n);
+
 #endif

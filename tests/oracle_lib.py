@@ -148,6 +148,102 @@ def reference():
     return _cache["ref"]
 
 
+class Dop:
+    """src/dop.c's API over either library: the oracle (or_*) or the
+    compiled reference (dop_* / bits_*), float +-1.0 streams."""
+
+    def __init__(self, lib_, prefix):
+        L = lib_.lib
+        n = (lambda s: "or_" + s) if prefix == "ora" else (lambda s: s)
+        self.f = {k: getattr(L, n(k)) for k in ("dop_unpack", "dop_pack", "dop_pack_i24", "bits_unpack",
+                                                  "bits_pack")}
+        self.det1 = getattr(L, "or_dop_detect" if prefix == "ora" else "dop_detect_interleaved")
+        self.f["dop_pack_i24"].restype = C.c_int
+        self.det1.restype = C.c_int if prefix == "ora" else C.c_bool
+
+    def detect(self, pcm, channels=1):
+        a = np.ascontiguousarray(pcm, dtype=np.float32)
+        return bool(self.det1(_ptr(a, _fp), C.c_size_t(a.size // channels), C.c_int(channels)))
+
+    def unpack(self, pcm):
+        a = np.ascontiguousarray(pcm, dtype=np.float32)
+        out = np.zeros(a.size * 16, np.float32)
+        self.f["dop_unpack"](_ptr(a, _fp), _ptr(out, _fp), C.c_size_t(a.size))
+        return out
+
+    def pack(self, bits):
+        b = np.ascontiguousarray(bits, dtype=np.float32)
+        out = np.zeros(b.size // 16, np.float32)
+        self.f["dop_pack"](_ptr(b, _fp), _ptr(out, _fp), C.c_size_t(b.size))
+        return out
+
+    def pack_i24(self, bits, phase=0):
+        b = np.ascontiguousarray(bits, dtype=np.float32)
+        out = np.zeros((b.size // 16) * 3, np.uint8)
+        r = self.f["dop_pack_i24"](_ptr(b, _fp), out.ctypes.data_as(C.c_void_p), C.c_size_t(b.size), C.c_int(phase))
+        return out, int(r)
+
+    def bits_unpack(self, src, n_bits):
+        s = np.ascontiguousarray(src, dtype=np.uint8)
+        out = np.zeros(n_bits, np.float32)
+        self.f["bits_unpack"](s.ctypes.data_as(C.c_void_p), _ptr(out, _fp), C.c_size_t(n_bits))
+        return out
+
+    def bits_pack(self, src):
+        a = np.ascontiguousarray(src, dtype=np.float32)
+        out = np.zeros((a.size + 7) // 8, np.uint8)
+        self.f["bits_pack"](_ptr(a, _fp), out.ctypes.data_as(C.c_void_p), C.c_size_t(a.size))
+        return out
+
+
+class OrCond(C.Structure):
+    _fields_ = [("taps", C.c_int), ("ring", C.c_double * 64), ("pos", C.c_int), ("sum", C.c_double),
+                ("lp_n", C.c_int), ("lp", C.c_double * 127), ("dly", C.c_double * 127)]
+
+
+class Cond:
+    """The engine's pre-SDM conditioning for one channel (or_cond.c)."""
+
+    def __init__(self, lib_, taps, lowpass_rate=0):
+        self.L = lib_.lib
+        self.c = OrCond()
+        self.L.or_cond_init(C.byref(self.c), C.c_int(taps), C.c_uint(lowpass_rate))
+
+    def _run(self, fn, x, *g):
+        a = np.ascontiguousarray(x, dtype=np.float32)
+        out = np.zeros(a.size)
+        args = [C.byref(self.c), _ptr(a, _fp), C.c_size_t(a.size)] + [C.c_double(v) for v in g] + [_ptr(out, _dp)]
+        getattr(self.L, fn)(*args)
+        return out
+
+    def dsd_boxcar(self, x, gs, ge):
+        return self._run("or_cond_dsd_boxcar", x, gs, ge)
+
+    def dsd_lowpass(self, x, gs, ge):
+        return self._run("or_cond_dsd_lowpass", x, gs, ge)
+
+    def pcm_boxcar(self, x):
+        return self._run("or_cond_pcm_boxcar", x)
+
+
+def lowpass_taps(lib_, rate):
+    h = np.zeros(127)
+    lib_.lib.or_lowpass_taps(C.c_uint(rate), _ptr(h, _dp))
+    return h
+
+
+def gain_ramp(lib_, x, gs, ge):
+    y = np.ascontiguousarray(x, dtype=np.float64).copy()
+    lib_.lib.or_gain_ramp(_ptr(y, _dp), C.c_size_t(y.size), C.c_double(gs), C.c_double(ge))
+    return y
+
+
+def soft_clip(lib_, x):
+    y = np.ascontiguousarray(x, dtype=np.float64).copy()
+    lib_.lib.or_soft_clip(_ptr(y, _dp), C.c_size_t(y.size))
+    return y
+
+
 def reference_fma():
     """The reference compiled with its shipped AVX2/FMA path selected, or None."""
     if "ref_fma" not in _cache:
