@@ -188,14 +188,14 @@ int dsdhip_fir_reserve(dsdhip_fir_t *f, size_t n)
     return 0;
 }
 
-int dsdhip_fir_process_dev(dsdhip_fir_t *f, const float *d_in, int64_t in_stride, size_t n,
-                           double *d_out, int64_t out_stride)
+static int fir_run(dsdhip_fir_t *f, const void *d_in, int in_f32, int64_t in_stride, size_t n, double *d_out,
+                   int64_t out_stride)
 {
     if (!f || !d_in || !d_out) return -1;
     if (n == 0) return 0;
     if (dsdhip_fir_reserve(f, n) != 0) return -1;
     const void *src = d_in;
-    int src_f32 = 1;
+    int src_f32 = in_f32;
     int64_t src_stride = in_stride;
     size_t cnt = n;
     for (int s = 0; s < f->nst; s++) {
@@ -216,6 +216,20 @@ int dsdhip_fir_process_dev(dsdhip_fir_t *f, const float *d_in, int64_t in_stride
         cnt *= 2;
     }
     return 0;
+}
+
+int dsdhip_fir_process_dev(dsdhip_fir_t *f, const float *d_in, int64_t in_stride, size_t n,
+                           double *d_out, int64_t out_stride)
+{
+    return fir_run(f, d_in, 1, in_stride, n, d_out, out_stride);
+}
+
+/* fir_chain_process_d with double input (the engine's boxcar-smoothed
+ * signal, src/engine.c:620-645). */
+int dsdhip_fir_process_f64_dev(dsdhip_fir_t *f, const double *d_in, int64_t in_stride, size_t n,
+                               double *d_out, int64_t out_stride)
+{
+    return fir_run(f, d_in, 0, in_stride, n, d_out, out_stride);
 }
 
 /* Host-buffer form: in nch x n floats, out nch x n*2^nst doubles. */

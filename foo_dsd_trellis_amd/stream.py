@@ -24,9 +24,27 @@ def _sigs(L):
     L.dsdhip_stream_ratio.argtypes = [C.c_void_p]
     L.dsdhip_stream_process_dev.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_size_t, C.c_void_p,
                                             C.c_int64, C.POINTER(C.c_size_t)]
+    L.dsdhip_stream_create_cfg.restype = C.c_void_p
+    L.dsdhip_stream_create_cfg.argtypes = [C.POINTER(StreamCfg)]
+    L.dsdhip_stream_set_gain.argtypes = [C.c_void_p, C.c_double]
+    L.dsdhip_stream_process_dsd_dev.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_size_t, C.c_void_p,
+                                                C.c_int64, C.POINTER(C.c_size_t)]
     L.dsdhip_stream_copy_signal.restype = C.c_int64
     L.dsdhip_stream_copy_signal.argtypes = [C.c_void_p, _dp, C.c_int64]
     L._stream_sigs = True
+
+
+IN_PCM, IN_DSD = 0, 1
+
+
+class StreamCfg(C.Structure):
+    """dsdhip_stream_cfg_t"""
+    _fields_ = [("fs_in", C.c_uint), ("fs_out", C.c_uint), ("nch", C.c_int), ("chunk_frames", C.c_size_t),
+                ("input", C.c_int), ("fir_gain", C.c_double), ("gain", C.c_double), ("pcm_boxcar", C.c_int),
+                ("soft_clip", C.c_int), ("dsd_boxcar", C.c_int), ("dsd_lowpass", C.c_int), ("ir", _dp),
+                ("ir_len", C.c_int64), ("ir_stride", C.c_int64), ("P", C.c_int), ("ntf_index", C.c_int),
+                ("depth", C.c_int), ("cands", C.c_int), ("lat", C.c_int), ("state_limit", C.c_double),
+                ("seg_len", C.c_size_t), ("reposition", C.c_int), ("fma_o8", C.c_int)]
 
 
 class Stream:
@@ -34,25 +52,43 @@ class Stream:
     ir: None or (nch, taps) float64 room impulse responses at fs_out."""
 
     def __init__(self, fs_in, fs_out, nch, chunk_frames, ir=None, P=8192, ntf_index=None, K=32, L=64,
-                 depth=8, gain=fir.PCM_FIR_GAIN, state_limit=0.0, seg_len=65536, reposition=True):
+                 depth=8, gain=fir.PCM_FIR_GAIN, state_limit=0.0, seg_len=65536, reposition=True,
+                 input=IN_PCM, volume=1.0, pcm_boxcar=0, soft_clip=False, dsd_boxcar=0, dsd_lowpass=False,
+                 fma_o8=False):
+        """gain: the engine's fir_gain; volume: cfg->gain (set_gain ramps it)."""
         self.lib = _lib.lib()
         _sigs(self.lib)
         if ntf_index is None:
             ntf_index = trellis.ntf_auto(fs_out)
         self.nch, self.chunk_frames = nch, chunk_frames
-        irp, ir_len = None, 0
+        c = StreamCfg()
+        c.fs_in, c.fs_out, c.nch, c.chunk_frames, c.input = fs_in, fs_out, nch, chunk_frames, input
+        c.fir_gain, c.gain = gain, volume
+        c.pcm_boxcar, c.soft_clip, c.dsd_boxcar, c.dsd_lowpass = pcm_boxcar, int(soft_clip), dsd_boxcar, int(
+            dsd_lowpass)
         if ir is not None:
             self._ir = np.ascontiguousarray(np.atleast_2d(ir), dtype=np.float64)
             assert self._ir.shape[0] == nch
-            irp, ir_len = self._ir.ctypes.data_as(_dp), self._ir.shape[1]
-        self.h = self.lib.dsdhip_stream_create(fs_in, fs_out, gain, nch, chunk_frames, irp, ir_len, ir_len,
-                                               P, ntf_index, depth, K, L, state_limit, seg_len,
-                                               1 if reposition else 0)
+            c.ir, c.ir_len, c.ir_stride = self._ir.ctypes.data_as(_dp), self._ir.shape[1], self._ir.shape[1]
+        c.P, c.ntf_index, c.depth, c.cands, c.lat = P, ntf_index, depth, K, L
+        c.state_limit, c.seg_len, c.reposition, c.fma_o8 = state_limit, seg_len, int(reposition), int(fma_o8)
+        self.h = self.lib.dsdhip_stream_create_cfg(C.byref(c))
         if not self.h:
-            raise _lib.DsdHipError("dsdhip_stream_create failed")
+            raise _lib.DsdHipError("dsdhip_stream_create_cfg failed")
         self.R = self.lib.dsdhip_stream_ratio(self.h)
         self.words = (chunk_frames * self.R + 63) // 64
         self.bits = torch.zeros((nch, self.words), dtype=torch.int64, device="cuda")
+
+    def set_gain(self, volume):
+        _lib.check(self.lib.dsdhip_stream_set_gain(self.h, volume), "dsdhip_stream_set_gain")
+
+    def process_dsd(self, bits, n):
+        """bits: cuda int64 (nch, words) packed DSD at fs_in, n samples per channel."""
+        cnt = C.c_size_t(0)
+        _lib.check(self.lib.dsdhip_stream_process_dsd_dev(self.h, bits.data_ptr(), bits.shape[1], n,
+                                                          self.bits.data_ptr(), self.words, C.byref(cnt)),
+                   "dsdhip_stream_process_dsd_dev")
+        return cnt.value
 
     def process(self, pcm):
         """pcm: cuda float32 (nch, frames) with frames <= chunk_frames.

@@ -181,6 +181,10 @@ int dsdhip_fir_process_dev(dsdhip_fir_t *f, const float *d_in, int64_t in_stride
                            double *d_out, int64_t out_stride);
 /* Host buffers: in nch x n floats, out nch x n*2^stages doubles. */
 int dsdhip_fir_process(dsdhip_fir_t *f, const float *in, size_t n, double *out);
+/* fir_chain_process_d with double input (the boxcar-smoothed signal,
+ * src/engine.c:620-645); device buffers. */
+int dsdhip_fir_process_f64_dev(dsdhip_fir_t *f, const double *d_in, int64_t in_stride, size_t n,
+                               double *d_out, int64_t out_stride);
 
 /* ------------------------------------------------------------------------
  * 5. DAS — src/dsp_plugin.c:1314-2761 (parallel trellis + stitching)
@@ -338,6 +342,38 @@ int dsdhip_pe_backward(const float *d_p, int B, size_t N, int ntaps, const doubl
  *    (src/engine.c:704-716) is not applied.
  * ====================================================================== */
 typedef struct dsdhip_stream dsdhip_stream_t;
+#define DSDHIP_IN_PCM 0   /* float32 PCM, [nch][stride] */
+#define DSDHIP_IN_DSD 1   /* packed DSD bits, [nch][words] */
+/* Engine-level configuration (the fields of dsd_config_t and the engine
+ * channel this chain needs, include/engine.h / src/engine.c:117-413). */
+typedef struct {
+    unsigned fs_in, fs_out;     /* fs_out == fs_in: same-rate DSD re-encode */
+    int nch;
+    size_t chunk_frames;        /* input samples per channel per chunk */
+    int input;                  /* DSDHIP_IN_PCM or DSDHIP_IN_DSD */
+    double fir_gain;            /* engine fir_gain (path table 0.708; PCM default 0.354) */
+    double gain;                /* volume (cfg->gain); changes ramp over the next chunk */
+    int pcm_boxcar;             /* PCM: boxcar taps before the FIR (engine_process_block: 8; 0 off) */
+    int soft_clip;              /* rate conversion: the engine's tanh soft clip (1 on) */
+    int dsd_boxcar;             /* DSD same-rate: boxcar taps (0: the engine's by rate);
+                                   DSD rate conversion: pre-smooth taps (0 off, engine: 8) */
+    int dsd_lowpass;            /* DSD same-rate: 127-tap lowpass instead of the boxcar */
+    const double *ir;           /* room IR [nch][ir_stride] at fs_out, or NULL */
+    int64_t ir_len, ir_stride;
+    int P;                      /* convolution partition */
+    int ntf_index, depth, cands, lat;
+    double state_limit;
+    size_t seg_len;
+    int reposition;
+    int fma_o8;                 /* order 8: the AVX2 build's FMA filter */
+} dsdhip_stream_cfg_t;
+dsdhip_stream_t *dsdhip_stream_create_cfg(const dsdhip_stream_cfg_t *cfg);
+/* volume: the next chunk ramps from the previous gain to this one */
+int dsdhip_stream_set_gain(dsdhip_stream_t *s, double gain);
+/* DSD input chunk: d_in nch rows of packed bits (words_stride words), n
+ * samples per channel (<= chunk_frames); output as for PCM. */
+int dsdhip_stream_process_dsd_dev(dsdhip_stream_t *s, const unsigned long long *d_in, int64_t words_stride,
+                                  size_t n, unsigned long long *d_bits, int64_t out_words, size_t *out_count);
 /* ir == NULL: no convolution.  Otherwise ir is host f64 [nch][ir_stride]
  * and chunk_frames * 2^stages must be a multiple of 8*P.  The DAS runs
  * all segments from the persistent contexts (seg0_first = 0) with segments

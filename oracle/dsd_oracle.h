@@ -129,6 +129,8 @@ void or_fir_stage(const double *h2, int ntaps, const double *hist, int nhist,
 size_t or_fir_chain(unsigned fs_in, unsigned fs_out, const float *in,
                     size_t n, double gain, double *out);
 /* Same chain in long double with the exact pairwise-free sum (truth). */
+size_t or_fir_chain_d(unsigned fs_in, unsigned fs_out, const double *in,
+                      size_t n, double gain, double *out);
 size_t or_fir_chain_ld(unsigned fs_in, unsigned fs_out, const float *in,
                        size_t n, double gain, long double *out);
 

@@ -114,6 +114,32 @@ size_t or_fir_chain(unsigned fs_in, unsigned fs_out, const float *in,
     return cnt;
 }
 
+/* The chain with double input (fir_chain_process_d of a boxcar-smoothed
+ * signal, src/engine.c:620-645). */
+size_t or_fir_chain_d(unsigned fs_in, unsigned fs_out, const double *in,
+                      size_t n, double gain, double *out)
+{
+    int ntaps[OR_FIR_MAX_STAGES];
+    int st = or_fir_plan(fs_in, fs_out, ntaps);
+    if (st < 0) return 0;
+    double *a = (double *)malloc(n * sizeof(double));
+    for (size_t i = 0; i < n; i++) a[i] = in[i];
+    size_t cnt = n;
+    for (int s = 0; s < st; s++) {
+        double h[255], h2[255];
+        or_design_halfband(h, ntaps[s], 12.0);
+        for (int k = 0; k < ntaps[s]; k++) h2[k] = h[k] * 2.0;
+        double *b = (s == st - 1) ? out : (double *)malloc(2 * cnt * sizeof(double));
+        or_fir_stage(h2, ntaps[s], NULL, 0, a, cnt, b);
+        free(a);
+        a = b;
+        cnt *= 2;
+    }
+    if (gain != 1.0)
+        for (size_t i = 0; i < cnt; i++) out[i] *= gain;
+    return cnt;
+}
+
 size_t or_fir_chain_ld(unsigned fs_in, unsigned fs_out, const float *in,
                        size_t n, double gain, long double *out)
 {

@@ -309,6 +309,17 @@ def fir_chain(lib_, fs_in, fs_out, x, gain=1.0, long_double=False):
     return out[:n]
 
 
+def fir_chain_d(lib_, fs_in, fs_out, x, gain=1.0):
+    """Oracle FIR chain of one channel with float64 input."""
+    L = lib_.lib
+    L.or_fir_chain_d.restype = C.c_size_t
+    L.or_fir_chain_d.argtypes = [C.c_uint, C.c_uint, _dp, C.c_size_t, C.c_double, _dp]
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(x.size * (fs_out // fs_in))
+    n = L.or_fir_chain_d(fs_in, fs_out, _ptr(x, _dp), x.size, gain, _ptr(out, _dp))
+    return out[:n]
+
+
 def halfband(lib_, ntaps):
     h = np.zeros(ntaps)
     lib_.lib.ora_halfband.argtypes = [_dp, C.c_int]
