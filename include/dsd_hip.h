@@ -467,6 +467,49 @@ int dsdhip_gain_ramp_dev(double *d_x, int64_t stride, int nch, size_t n, double 
 /* soft clip in place above +-0.95 with a tanh knee (:704-716) */
 int dsdhip_soft_clip_dev(double *d_x, int64_t stride, int nch, size_t n, void *stream);
 
+/* ======================================================================
+ * 12. PreCorr SDM (src/precorr.c, include/precorr.h): greedy 1-bit
+ *     quantizer through the NTF loop filter in single precision plus the
+ *     8-bit-history x 8-phase prediction-table correction.  No latency.
+ *     One sequential recurrence per channel on the device.
+ * ====================================================================== */
+/* Same layout as gpu_precorr_state_t (include/gpu_compute.h:128-136). */
+typedef struct {
+    float state[8];
+    float prev_y;
+    int history;
+    int phase;
+    float pad;
+} dsdhip_precorr_state_t;
+typedef dsdhip_precorr_state_t gpu_hip_precorr_state_t;
+/* precorr_train_table (src/precorr.c:134-186) for float coefficients:
+ * table[256][8] of mean corrections. */
+int dsdhip_precorr_train(const float *a, const float *g, int order, float *table);
+typedef struct dsdhip_precorr dsdhip_precorr_t;
+/* replaces precorr_context_init (include/precorr.h:40) for nch channels;
+ * state_limit as precorr_context_t.state_limit (0 = off) */
+dsdhip_precorr_t *dsdhip_precorr_create(int ntf_index, int nch, float state_limit);
+dsdhip_precorr_t *dsdhip_precorr_create_coeffs(const double *a, const double *g, int order, int nch,
+                                               float state_limit);
+void dsdhip_precorr_destroy(dsdhip_precorr_t *p);
+/* replaces precorr_context_reset (include/precorr.h:49) on every channel */
+int dsdhip_precorr_reset(dsdhip_precorr_t *p);
+int dsdhip_precorr_set_stream(dsdhip_precorr_t *p, void *hip_stream);
+int dsdhip_precorr_table(const dsdhip_precorr_t *p, float *table);
+int dsdhip_precorr_get_state(dsdhip_precorr_t *p, int ch, dsdhip_precorr_state_t *st);
+int dsdhip_precorr_set_state(dsdhip_precorr_t *p, int ch, const dsdhip_precorr_state_t *st);
+/* replaces precorr_process_block (include/precorr.h:43) for every channel:
+ * d_in [nch][in_stride] doubles, n samples each -> packed bits
+ * [nch][words_stride] (n out per channel, no latency) */
+int dsdhip_precorr_process_dev(dsdhip_precorr_t *p, const double *d_in, int64_t in_stride, size_t n,
+                               unsigned long long *d_bits, int64_t words_stride);
+/* replaces gpu_cuda_precorr_process (include/gpu_compute.h:140-144): one
+ * channel, host buffers, float coefficients and table from the caller,
+ * state in and out; outputs +-1.0.  No state limit (the seam has none). */
+int gpu_hip_precorr_process(void *ctx, const float *in, float *out, size_t count, const float *ntf_a,
+                            const float *ntf_g, int order, const float pred_table[256][8],
+                            const gpu_hip_precorr_state_t *init, gpu_hip_precorr_state_t *final_state);
+
 /* Library version string. */
 const char *dsdhip_version(void);
 

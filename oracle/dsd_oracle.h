@@ -205,4 +205,19 @@ void or_cond_dsd_lowpass(or_cond *c, const float *in, size_t n, double gs, doubl
 void or_gain_ramp(double *x, size_t n, double gs, double ge);
 void or_soft_clip(double *x, size_t n);
 
+/* PreCorr SDM (or_precorr.c), one channel. */
+typedef struct {
+    float a[8], g[8];
+    int order;
+    float state_limit;
+    float table[256][8];
+    float state[OR_MAX_ORDER];
+    unsigned history;
+    int phase;
+    float prev_y;
+} or_pc;
+int or_pc_init(or_pc *c, int filter_index, float state_limit);
+void or_pc_reset(or_pc *c);
+size_t or_pc_process(or_pc *c, const double *in, float *out, size_t n);
+
 #endif

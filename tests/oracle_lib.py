@@ -244,6 +244,65 @@ def soft_clip(lib_, x):
     return y
 
 
+class OrPc(C.Structure):
+    _fields_ = [("a", C.c_float * 8), ("g", C.c_float * 8), ("order", C.c_int), ("state_limit", C.c_float),
+                ("table", C.c_float * 2048), ("state", C.c_float * 10), ("history", C.c_uint), ("phase", C.c_int),
+                ("prev_y", C.c_float)]
+
+
+class PreCorr:
+    """precorr_context_t over the oracle ("ora") or the compiled reference ("ref")."""
+
+    def __init__(self, lib_, prefix, filter_index, state_limit=0.0):
+        self.L = lib_.lib
+        self.p = prefix
+        if prefix == "ora":
+            self.c = OrPc()
+            self.L.or_pc_init(C.byref(self.c), C.c_int(filter_index), C.c_float(state_limit))
+        else:
+            self.L.ref_pc_new.restype = C.c_void_p
+            self.L.ref_pc_new.argtypes = [C.c_int, C.c_float]
+            for f in ("ref_pc_free", "ref_pc_reset"):
+                getattr(self.L, f).argtypes = [C.c_void_p]
+            self.L.ref_pc_process.argtypes = [C.c_void_p, _dp, _fp, C.c_size_t]
+            self.L.ref_pc_table.argtypes = [C.c_void_p, _fp]
+            self.L.ref_pc_state.argtypes = [C.c_void_p, _fp, C.POINTER(C.c_int), C.POINTER(C.c_int), _fp]
+            self.h = self.L.ref_pc_new(filter_index, state_limit)
+
+    def process(self, x):
+        a = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(a.size, np.float32)
+        if self.p == "ora":
+            self.L.or_pc_process(C.byref(self.c), _ptr(a, _dp), _ptr(out, _fp), C.c_size_t(a.size))
+        else:
+            self.L.ref_pc_process(self.h, _ptr(a, _dp), _ptr(out, _fp), a.size)
+        return out
+
+    def table(self):
+        if self.p == "ora":
+            return np.ctypeslib.as_array(self.c.table).copy()
+        t = np.zeros(2048, np.float32)
+        self.L.ref_pc_table(self.h, _ptr(t, _fp))
+        return t
+
+    def state(self):
+        """(state[8], history, phase, prev_y)"""
+        if self.p == "ora":
+            return (np.ctypeslib.as_array(self.c.state)[:8].copy(), int(self.c.history), int(self.c.phase),
+                    float(self.c.prev_y))
+        st = np.zeros(8, np.float32)
+        h, ph, py = C.c_int(0), C.c_int(0), (C.c_float * 1)()
+        self.L.ref_pc_state(self.h, _ptr(st, _fp), C.byref(h), C.byref(ph), py)
+        return st, int(h.value), int(ph.value), float(py[0])
+
+    def __del__(self):
+        try:
+            if self.p == "ref" and self.h:
+                self.L.ref_pc_free(self.h)
+        except Exception:
+            pass
+
+
 def reference_fma():
     """The reference compiled with its shipped AVX2/FMA path selected, or None."""
     if "ref_fma" not in _cache:
