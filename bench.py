@@ -252,6 +252,79 @@ def streaming_room(chunks, seg_len):
                       "room IR (P=8192, c-f32) -> DAS K=32 L=64 B=8, one stream, chunks in order"}
 
 
+def next_rows(d_bits, n_bits, seconds):
+    """The rows after the hot path, on data of the headline workload:
+    PreCorr (greedy SDM, one sequential recurrence per channel) over 6
+    channels of DSD256-rate input; the same-rate DSD256 re-encode of the
+    headline's own output bits (the engine's boxcar of 64 taps and path
+    settings CLANS-6, K=2, L=16, B=4, DAS with 16,384-sample segments); DoP
+    packing and unpacking of those bits (HBM-bound byte work)."""
+    import torch
+    from foo_dsd_trellis_amd import dop, precorr, signals, stream, trellis
+    res = {}
+    # PreCorr: 6 channels x `seconds` at DSD256
+    n = int(seconds * DSD256)
+    t = torch.arange(n, dtype=torch.float64, device="cuda") / DSD256
+    x = torch.stack([0.4 * torch.sin(2 * np.pi * (997.0 + 200 * c) * t) for c in range(6)]).contiguous()
+    fi = trellis.ntf_auto(DSD256)
+    pc = precorr.PreCorr(fi, 6, 0.0)
+    pc.process(x[:, :65536])
+    pc.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pc.process(x)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    res["precorr"] = {"value": round(6 * n / dt / 1e6, 3), "unit": "Msamples/s", "channels": 6,
+                      "samples_per_channel": n, "x_realtime": round(seconds / dt, 3),
+                      "note": "greedy SDM is sequential per channel: one wave per channel"}
+    del x, t
+    # same-rate DSD256 re-encode of the headline output (first `seconds` per channel)
+    m = min(n_bits, n) // 65536 * 65536
+    cf = 1 << 20
+    m = max(cf, m // cf * cf)
+    words = m // 64
+    src = d_bits[:, :words].contiguous()
+    fi6 = trellis.ntf_lookup(4, DSD256)            # clans-6 (the path table's DSD256 same-rate filter)
+    S = stream.Stream(DSD256, DSD256, 2, cf, ntf_index=fi6, K=2, L=16, depth=4, seg_len=16384,
+                      gain=float(np.float32(0.708)), input=stream.IN_DSD)
+    S.process_dsd(src[:, :cf // 64], cf)
+    S.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tot = 0
+    for k in range(m // cf):
+        tot += S.process_dsd(src[:, k * cf // 64:(k + 1) * cf // 64], cf)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    res["dsd_same_rate_reencode"] = {"value": round(2 * tot / dt / 1e6, 3), "unit": "Msamples/s",
+                                     "x_realtime": round(m / DSD256 / dt, 3), "channels": 2,
+                                     "chunk": cf, "config": "DSD256 -> DSD256, boxcar 64, CLANS-6 K=2 L=16 B=4, DAS"}
+    # DoP pack (i24, interleaved) and unpack of the same bits
+    nb = words * 64
+    out, _ = dop.pack_i24(src, nb, 0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        out, _ = dop.pack_i24(src, nb, 0)
+    torch.cuda.synchronize()
+    dp = (time.perf_counter() - t0) / 5
+    pcm = dop.pack(src, nb)
+    back = dop.unpack(pcm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        back = dop.unpack(pcm)
+    torch.cuda.synchronize()
+    du = (time.perf_counter() - t0) / 5
+    ok = bool(torch.equal(back[:, :words], src))
+    res["dop"] = {"pack_i24_GBps": round((src.numel() * 8 + out.numel()) / dp / 1e9, 2),
+                  "unpack_GBps": round((pcm.numel() * 4 + back.numel() * 8) / du / 1e9, 2),
+                  "roundtrip_exact": ok, "frames_per_channel": nb // 16,
+                  "note": "bytes read + written per call over wall time; HBM peak 8000 GB/s"}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,6 +347,8 @@ def main():
     ap.add_argument("--stream-chunks", type=int, default=30,
                     help="timed chunks of the streaming room-correction workload (0: skip)")
     ap.add_argument("--stream-seg-len", type=int, default=8192)
+    ap.add_argument("--next-seconds", type=float, default=0.5,
+                    help="audio seconds per channel for the PreCorr / same-rate / DoP legs (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
@@ -357,6 +432,7 @@ def main():
     bu = (batched_uneven(args.batch_chunks, args.batch_chunk_frames, args.batch_plain_tracks)
           if args.batch_chunks > 1 else None)
     sr = streaming_room(args.stream_chunks, args.stream_seg_len) if args.stream_chunks > 0 else None
+    nr = next_rows(d_bits, n_out, args.next_seconds) if args.next_seconds > 0 else None
 
     line = {
         "metric": "Trellis SDM Msamples/s & xrealtime at DSD256 K=32, batched vs plain",
@@ -376,6 +452,7 @@ def main():
         "diff_loop_filter": lf,
         "batched_uneven": bu,
         "streaming_room": sr,
+        "next_rows": nr,
     }
     print(json.dumps(line), flush=True)
 

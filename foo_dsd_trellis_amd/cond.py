@@ -58,13 +58,13 @@ class Conditioner:
 
     def dsd_boxcar(self, bits, n, gs, ge):
         out = self._out(n, bits)
-        _lib.check(self.L.dsdhip_cond_dsd_boxcar_dev(self.h, bits.data_ptr(), bits.shape[1], n, gs, ge,
+        _lib.check(self.L.dsdhip_cond_dsd_boxcar_dev(self.h, bits.data_ptr(), bits.stride(0), n, gs, ge,
                                                      out.data_ptr(), n), "dsdhip_cond_dsd_boxcar_dev")
         return out
 
     def dsd_lowpass(self, bits, n, gs, ge):
         out = self._out(n, bits)
-        _lib.check(self.L.dsdhip_cond_dsd_lowpass_dev(self.h, bits.data_ptr(), bits.shape[1], n, gs, ge,
+        _lib.check(self.L.dsdhip_cond_dsd_lowpass_dev(self.h, bits.data_ptr(), bits.stride(0), n, gs, ge,
                                                       out.data_ptr(), n), "dsdhip_cond_dsd_lowpass_dev")
         return out
 

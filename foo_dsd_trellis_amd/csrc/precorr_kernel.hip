@@ -36,6 +36,10 @@ __global__ __launch_bounds__(64) void precorr_k(const TIn *__restrict__ in, int6
     int history = st->history & 255, phase = st->phase & 7;
     const float lim = prm.state_limit;
     u64 *o = out + (int64_t)c * out_stride;
+    /* the correction of the current step; the next step's is read for both
+     * possible output bits while this step's filter runs, so the table's
+     * LDS latency stays off the sample-to-sample dependence */
+    float corr = tab[history * 8 + phase];
     float xv = (lane < n) ? (float)x[lane] : 0.0f;
     for (int64_t b = 0; b < n; b += 64) {
         const int64_t nx = b + 64 + lane;
@@ -43,6 +47,8 @@ __global__ __launch_bounds__(64) void precorr_k(const TIn *__restrict__ in, int6
         const int m = (n - b < 64) ? (int)(n - b) : 64;
         u64 bits = 0;
         for (int j = 0; j < m; j++) {
+            const int hb = (history << 1) & 255, np = (phase + 1) & 7;
+            const float c0 = tab[hb * 8 + np], c1 = tab[(hb | 1) * 8 + np];
             const float xs = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xv), j));
             float d[P];
             d[0] = s[0] - g[0] * s[1] + xs - prev_y;
@@ -61,11 +67,12 @@ __global__ __launch_bounds__(64) void precorr_k(const TIn *__restrict__ in, int6
             }
 #pragma unroll
             for (int k = 0; k < P; k++) s[k] = d[k];
-            const float yc = y - tab[history * 8 + phase];
+            const float yc = y - corr;
             const int ob = (yc >= 0.0f) ? 1 : 0;
             bits |= (u64)ob << j;
-            history = ((history << 1) | ob) & 255;
-            phase = (phase + 1) & 7;
+            history = hb | ob;
+            phase = np;
+            corr = ob ? c1 : c0;
             prev_y = ob ? 1.0f : -1.0f;
         }
         if (lane == 0) o[b >> 6] = bits;

@@ -47,8 +47,9 @@ def words(n_bits):
 def unpack(pcm24):
     """cuda float32 (frames, nch) interleaved DoP -> int64 (nch, words) packed bits."""
     frames, nch = pcm24.shape
+    src = pcm24.contiguous()
     out = torch.zeros((nch, words(frames * 16)), dtype=torch.int64, device=pcm24.device)
-    _lib.check(_L().dsdhip_dop_unpack_dev(pcm24.data_ptr(), frames, nch, out.data_ptr(), out.shape[1], _stream()),
+    _lib.check(_L().dsdhip_dop_unpack_dev(src.data_ptr(), frames, nch, out.data_ptr(), out.shape[1], _stream()),
                "dsdhip_dop_unpack_dev")
     return out
 
@@ -57,7 +58,7 @@ def pack(bits, bit_count):
     """packed bits (nch, words) -> cuda float32 (bit_count // 16, nch) DoP frames."""
     nch = bits.shape[0]
     out = torch.empty((bit_count // 16, nch), dtype=torch.float32, device=bits.device)
-    _lib.check(_L().dsdhip_dop_pack_dev(bits.data_ptr(), bits.shape[1], nch, bit_count, out.data_ptr(), _stream()),
+    _lib.check(_L().dsdhip_dop_pack_dev(bits.data_ptr(), bits.stride(0), nch, bit_count, out.data_ptr(), _stream()),
                "dsdhip_dop_pack_dev")
     return out
 
@@ -66,7 +67,7 @@ def pack_i24(bits, bit_count, phase=0):
     """packed bits -> (uint8 (frames, nch, 3), next marker phase)."""
     nch = bits.shape[0]
     out = torch.empty((bit_count // 16, nch, 3), dtype=torch.uint8, device=bits.device)
-    r = _L().dsdhip_dop_pack_i24_dev(bits.data_ptr(), bits.shape[1], nch, bit_count, phase, out.data_ptr(),
+    r = _L().dsdhip_dop_pack_i24_dev(bits.data_ptr(), bits.stride(0), nch, bit_count, phase, out.data_ptr(),
                                      _stream())
     if r < 0:
         raise _lib.DsdHipError("dsdhip_dop_pack_i24_dev failed")
@@ -77,7 +78,7 @@ def bytes_unpack(src, n_bits):
     """uint8 (nch, nbytes) MSB-first -> packed bits."""
     nch = src.shape[0]
     out = torch.zeros((nch, words(n_bits)), dtype=torch.int64, device=src.device)
-    _lib.check(_L().dsdhip_bytes_unpack_dev(src.data_ptr(), src.shape[1], nch, n_bits, out.data_ptr(), out.shape[1],
+    _lib.check(_L().dsdhip_bytes_unpack_dev(src.data_ptr(), src.stride(0), nch, n_bits, out.data_ptr(), out.shape[1],
                                             _stream()), "dsdhip_bytes_unpack_dev")
     return out
 
@@ -85,7 +86,7 @@ def bytes_unpack(src, n_bits):
 def bytes_pack(bits, n_bits):
     nch = bits.shape[0]
     out = torch.empty((nch, (n_bits + 7) // 8), dtype=torch.uint8, device=bits.device)
-    _lib.check(_L().dsdhip_bytes_pack_dev(bits.data_ptr(), bits.shape[1], nch, n_bits, out.data_ptr(), out.shape[1],
+    _lib.check(_L().dsdhip_bytes_pack_dev(bits.data_ptr(), bits.stride(0), nch, n_bits, out.data_ptr(), out.shape[1],
                                           _stream()), "dsdhip_bytes_pack_dev")
     return out
 
@@ -94,6 +95,6 @@ def to_f32(bits, n):
     """packed bits -> cuda float32 (nch, n) of +-1.0."""
     nch = bits.shape[0]
     out = torch.empty((nch, n), dtype=torch.float32, device=bits.device)
-    _lib.check(_L().dsdhip_bits_to_f32_dev(bits.data_ptr(), bits.shape[1], nch, n, out.data_ptr(), n, _stream()),
+    _lib.check(_L().dsdhip_bits_to_f32_dev(bits.data_ptr(), bits.stride(0), nch, n, out.data_ptr(), n, _stream()),
                "dsdhip_bits_to_f32_dev")
     return out

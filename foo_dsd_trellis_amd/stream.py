@@ -85,7 +85,7 @@ class Stream:
     def process_dsd(self, bits, n):
         """bits: cuda int64 (nch, words) packed DSD at fs_in, n samples per channel."""
         cnt = C.c_size_t(0)
-        _lib.check(self.lib.dsdhip_stream_process_dsd_dev(self.h, bits.data_ptr(), bits.shape[1], n,
+        _lib.check(self.lib.dsdhip_stream_process_dsd_dev(self.h, bits.data_ptr(), bits.stride(0), n,
                                                           self.bits.data_ptr(), self.words, C.byref(cnt)),
                    "dsdhip_stream_process_dsd_dev")
         return cnt.value
