@@ -257,7 +257,8 @@ def next_rows(d_bits, n_bits, seconds):
     PreCorr (greedy SDM, one sequential recurrence per channel) over 6
     channels of DSD256-rate input; the same-rate DSD256 re-encode of the
     headline's own output bits (the engine's boxcar of 64 taps and path
-    settings CLANS-6, K=2, L=16, B=4, DAS with 16,384-sample segments); DoP
+    settings CLANS-6, K=2, L=16, B=4, DAS with 4,096-sample segments),
+    streamed and offline; DoP
     packing and unpacking of those bits (HBM-bound byte work)."""
     import torch
     from foo_dsd_trellis_amd import dop, precorr, signals, stream, trellis
@@ -279,28 +280,35 @@ def next_rows(d_bits, n_bits, seconds):
                       "samples_per_channel": n, "x_realtime": round(seconds / dt, 3),
                       "note": "greedy SDM is sequential per channel: one wave per channel"}
     del x, t
-    # same-rate DSD256 re-encode of the headline output (first `seconds` per channel)
-    m = min(n_bits, n) // 65536 * 65536
-    cf = 1 << 20
-    m = max(cf, m // cf * cf)
-    words = m // 64
-    src = d_bits[:, :words].contiguous()
+    # same-rate DSD256 re-encode of the headline output: streamed in chunks
+    # of 2^20 samples (first `seconds` per channel), and converted offline in
+    # one chunk of 2^24 samples (1.49 s); K=2 runs one lane per segment job
     fi6 = trellis.ntf_lookup(4, DSD256)            # clans-6 (the path table's DSD256 same-rate filter)
-    S = stream.Stream(DSD256, DSD256, 2, cf, ntf_index=fi6, K=2, L=16, depth=4, seg_len=16384,
-                      gain=float(np.float32(0.708)), input=stream.IN_DSD)
-    S.process_dsd(src[:, :cf // 64], cf)
-    S.reset()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    tot = 0
-    for k in range(m // cf):
-        tot += S.process_dsd(src[:, k * cf // 64:(k + 1) * cf // 64], cf)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    res["dsd_same_rate_reencode"] = {"value": round(2 * tot / dt / 1e6, 3), "unit": "Msamples/s",
-                                     "x_realtime": round(m / DSD256 / dt, 3), "channels": 2,
-                                     "chunk": cf, "config": "DSD256 -> DSD256, boxcar 64, CLANS-6 K=2 L=16 B=4, DAS"}
-    # DoP pack (i24, interleaved) and unpack of the same bits
+    for name, cf, m in (("dsd_same_rate_reencode", 1 << 20, min(n_bits, n)),
+                        ("dsd_same_rate_reencode_offline", 1 << 24, min(n_bits, 1 << 24))):
+        m = max(cf, m // cf * cf)
+        if m > n_bits:
+            continue
+        words = m // 64
+        src = d_bits[:, :words].contiguous()
+        S = stream.Stream(DSD256, DSD256, 2, cf, ntf_index=fi6, K=2, L=16, depth=4, seg_len=4096,
+                          gain=float(np.float32(0.708)), input=stream.IN_DSD)
+        S.process_dsd(src[:, :cf // 64], cf)
+        S.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tot = 0
+        for k in range(m // cf):
+            tot += S.process_dsd(src[:, k * cf // 64:(k + 1) * cf // 64], cf)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res[name] = {"value": round(2 * tot / dt / 1e6, 3), "unit": "Msamples/s",
+                     "x_realtime": round(m / DSD256 / dt, 3), "channels": 2, "chunk": cf,
+                     "config": "DSD256 -> DSD256, boxcar 64, CLANS-6 K=2 L=16 B=4, DAS 4096-sample segments"}
+        del S, src
+    # DoP pack (i24, interleaved) and unpack of the first `seconds` of bits
+    words = max(1 << 14, min(n_bits, n) // 65536 * 1024)
+    src = d_bits[:, :words].contiguous()
     nb = words * 64
     out, _ = dop.pack_i24(src, nb, 0)
     torch.cuda.synchronize()

@@ -45,6 +45,8 @@ _SIGS = {
     "dsdhip_trellis_force_literal": ([_vp, C.c_int], C.c_int),
     "dsdhip_trellis_fma_o8": ([_vp, C.c_int], C.c_int),
     "dsdhip_das_fma_o8": ([_vp, C.c_int], C.c_int),
+    "dsdhip_trellis_wave_only": ([_vp, C.c_int], C.c_int),
+    "dsdhip_das_wave_only": ([_vp, C.c_int], C.c_int),
     "gpu_hip_trellis_fma_o8": ([_vp, C.c_int], C.c_int),
     "dsdhip_fir_plan": ([C.c_uint, C.c_uint, _ip], C.c_int),
     "dsdhip_halfband_taps": ([_dp, C.c_int], None),

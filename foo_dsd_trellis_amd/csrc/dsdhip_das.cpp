@@ -212,6 +212,7 @@ int dsdhip_das_set_stream(dsdhip_das_t *d, void *stream)
 
 int dsdhip_das_force_literal(dsdhip_das_t *d, int on) { return d ? dsdhip_trellis_force_literal(d->tr, on) : -1; }
 int dsdhip_das_fma_o8(dsdhip_das_t *d, int on) { return d ? dsdhip_trellis_fma_o8(d->tr, on) : -1; }
+int dsdhip_das_wave_only(dsdhip_das_t *d, int on) { return d ? dsdhip_trellis_wave_only(d->tr, on) : -1; }
 
 /* Plan for F samples: returns the segment count; fills start/size if given. */
 int dsdhip_das_plan(const dsdhip_das_t *d, size_t F, size_t *start, size_t *size)
@@ -261,6 +262,7 @@ static int ensure_pool(dsdhip_das_t *d, int S)
         dsdhip_trellis_params(d->tr, &op);
         dsdhip_trellis_force_literal(nt, op.force_slow);
         dsdhip_trellis_fma_o8(nt, op.fma_o8);
+        dsdhip_trellis_wave_only(nt, op.wave_only);
     }
     for (int s = 0; s < d->ns; s++) {
         tr_ctx c;

@@ -156,6 +156,13 @@ int dsdhip_trellis_fma_o8(dsdhip_trellis_t *t, int on)
     return 0;
 }
 
+int dsdhip_trellis_wave_only(dsdhip_trellis_t *t, int on)
+{
+    if (!t) return -1;
+    t->prm.wave_only = on ? 1 : 0;
+    return 0;
+}
+
 int dsdhip_trellis_num_contexts(const dsdhip_trellis_t *t) { return t ? t->nctx : 0; }
 
 void *dsdhip_trellis_context_ptr(dsdhip_trellis_t *t, int i)

@@ -762,11 +762,18 @@ template __global__ void tr_run_jobs_f8<8>(tr_params, tr_job *, tr_ctx *);
 template __global__ void tr_run_jobs_f8<16>(tr_params, tr_job *, tr_ctx *);
 template __global__ void tr_run_jobs_f8<32>(tr_params, tr_job *, tr_ctx *);
 
-/* Host-side launcher table (P in 4..8, HW in {1,2,4,8,16,32}). */
+extern "C" int tr_lane_eligible(const tr_params *prm);
+extern "C" int tr_lane_launch(const tr_params *prm, tr_job *d_jobs, int njobs, tr_ctx *d_ctxs,
+                              hipStream_t stream);
+
+/* Host-side launcher table (P in 4..8, HW in {1,2,4,8,16,32}).  At K <= 2
+ * the jobs run one per lane (tr_lane_kernel.hip) unless prm asks for the
+ * one-wave-per-job kernel. */
 extern "C" int tr_launch_jobs(const tr_params *prm, tr_job *d_jobs, int njobs,
                               tr_ctx *d_ctxs, hipStream_t stream)
 {
     if (njobs <= 0) return 0;
+    if (tr_lane_eligible(prm)) return tr_lane_launch(prm, d_jobs, njobs, d_ctxs, stream);
     const int P = prm->P;
     const int hw_need = (prm->L + 63) / 64;
     int HW = 1;
@@ -790,11 +797,13 @@ extern "C" int tr_launch_jobs(const tr_params *prm, tr_job *d_jobs, int njobs,
     return -1;
 }
 
-/* Persistent launch: nblocks workgroups of 64*wpb threads over njobs slots. */
+/* Persistent launch: nblocks workgroups of 64*wpb threads over njobs slots.
+ * At K <= 2 the slots run one per lane instead (the counter is not used). */
 extern "C" int tr_launch_queue(const tr_params *prm, tr_job *d_jobs, int njobs, int *d_counter,
                                tr_ctx *d_ctxs, int nblocks, int wpb, hipStream_t stream)
 {
     if (njobs <= 0) return 0;
+    if (tr_lane_eligible(prm)) return tr_lane_launch(prm, d_jobs, njobs, d_ctxs, stream);
     const int P = prm->P;
     const int hw_need = (prm->L + 63) / 64;
     int HW = 1;

@@ -48,6 +48,7 @@ typedef struct tr_params {
     uint32_t mask;
     int32_t force_slow;     /* validation: decide every step with the literal prune */
     int32_t fma_o8;         /* order 8: the 4-candidate FMA filter of the AVX2 build */
+    int32_t wave_only;      /* K <= 2 too on the one-wave-per-job kernel (no lane-per-job form) */
 } tr_params;
 
 /* One unit of trellis work: run `n_in` inputs through a context.

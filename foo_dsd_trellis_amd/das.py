@@ -43,6 +43,10 @@ class Das:
         """Order 8: the reference AVX2 build's 4-candidate FMA filter."""
         check(lib().dsdhip_das_fma_o8(self.h, 1 if on else 0), "fma_o8")
 
+    def wave_only(self, on=True):
+        """K <= 2: run each job on a whole wave instead of one lane."""
+        check(lib().dsdhip_das_wave_only(self.h, 1 if on else 0), "wave_only")
+
     def nseg(self, F):
         return lib().dsdhip_das_plan(self.h, F, None, None)
 

@@ -65,6 +65,10 @@ class SdmPool:
         """Order 8: the reference AVX2 build's 4-candidate FMA filter."""
         check(lib().dsdhip_trellis_fma_o8(self.h, 1 if on else 0), "fma_o8")
 
+    def wave_only(self, on=True):
+        """K <= 2: run each job on a whole wave instead of one lane."""
+        check(lib().dsdhip_trellis_wave_only(self.h, 1 if on else 0), "wave_only")
+
     def reset(self, i):
         check(lib().dsdhip_trellis_reset(self.h, i), "reset")
 

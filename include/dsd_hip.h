@@ -147,6 +147,10 @@ int dsdhip_trellis_force_literal(dsdhip_trellis_t *t, int on);
 /* Order 8: the AVX2 build's 4-candidate FMA filter (src/trellis.c:348-533,
  * 904-927) instead of the scalar one (see gpu_hip_trellis_fma_o8). */
 int dsdhip_trellis_fma_o8(dsdhip_trellis_t *t, int on);
+/* At K <= 2 (the engine's DSD-input paths, src/engine.c:43-103) jobs run one
+ * per lane, 64 per wave; on = 1 runs them one per wave like K > 2 (same
+ * bits; for validation and timing).  No reference counterpart. */
+int dsdhip_trellis_wave_only(dsdhip_trellis_t *t, int on);
 /* HIP-event timing of trellis launches on the pool's stream (no reference
  * counterpart; the reference times phases with QueryPerformanceCounter,
  * src/dsp_plugin.c:90-94). */
@@ -204,6 +208,7 @@ void dsdhip_das_destroy(dsdhip_das_t *d);
 int dsdhip_das_set_stream(dsdhip_das_t *d, void *hip_stream);
 int dsdhip_das_force_literal(dsdhip_das_t *d, int on);
 int dsdhip_das_fma_o8(dsdhip_das_t *d, int on);
+int dsdhip_das_wave_only(dsdhip_das_t *d, int on);
 /* Segment count (and nominal starts/sizes if given) for F samples. */
 int dsdhip_das_plan(const dsdhip_das_t *d, size_t F, size_t *start, size_t *size);
 /* One chunk: d_in nch x F doubles (row stride in_stride); d_out nch rows
