@@ -95,6 +95,18 @@ __device__ __forceinline__ unsigned shl_add_lt(unsigned acc, double k, double c)
     return r;
 }
 
+/* acc * 2 + (k <= c) */
+__device__ __forceinline__ unsigned shl_add_le(unsigned acc, double k, double c)
+{
+    unsigned r;
+    __asm__("v_cmp_le_f64_e32 vcc, %1, %2\n\t"
+            "v_addc_co_u32_e32 %0, vcc, %3, %3, vcc"
+            : "=v"(r)
+            : "v"(k), "v"(c), "v"(acc)
+            : "vcc");
+    return r;
+}
+
 /* acc * 2 + (k < c) for unsigned 32-bit keys */
 __device__ __forceinline__ unsigned shl_add_ltu(unsigned acc, unsigned k, unsigned c)
 {
@@ -160,6 +172,29 @@ __device__ __forceinline__ u64 lt_sweep(const double *key, double c)
     return ((u64)w[1] << 32) | w[0];
 }
 
+/* Order mask "at most c" against the first NJ metrics, read like lt_sweep. */
+template <int NJ>
+__device__ __forceinline__ u64 le_sweep(const double *key, double c)
+{
+    const double2 *k2 = reinterpret_cast<const double2 *>(key);
+    unsigned w[2] = {0u, 0u};
+#pragma unroll
+    for (int b = NJ / 8 - 1; b >= 0; b--) {
+        double2 kv[4];
+#pragma unroll
+        for (int i = 3; i >= 0; i--) kv[i] = k2[b * 4 + i];
+        unsigned acc = w[b >> 2];
+#pragma unroll
+        for (int i = 3; i >= 0; i--) {
+            acc = shl_add_le(acc, kv[i].y, c);
+            acc = shl_add_le(acc, kv[i].x, c);
+        }
+        w[b >> 2] = acc;
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    return ((u64)w[1] << 32) | w[0];
+}
+
 /* LDS working set of one context: the per-slot records of both banks
  * (followed by the path-hash chains), the prune's tables, and the lane
  * holding each survivor.  The survivors' own records (loop-filter state,
@@ -175,7 +210,6 @@ struct TrLds {
     u64 btab[128 + 64];
     int8_t link[2][64];
     uint8_t rank[64];
-    uint8_t nnext[64];
     int8_t head[128];
     int8_t act[TR_MAXK + 1];
     int8_t src[TR_MAXK];           /* lane holding the survivor of each rank */
@@ -198,40 +232,63 @@ __device__ __forceinline__ u64 bp_u64(u64 v, int from)
     return ((u64)hi << 32) | lo;
 }
 
-/* Literal sequential prune (src/trellis.c:582-731), run by one lane over the
- * children held in LDS.  Writes rank[] (255 = pruned) and returns the number
- * of survivors; updates hash links of newly registered children. */
-template <int P, int HW>
-__device__ int literal_prune(TrLds<P, HW> &s, int nb, int m, int K,
-                             const tr_ctx *snap, int *err)
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ long long rl_ll(long long v, int l)
 {
-    const uint32_t *np = s.npath[nb];
-    const double *nc = s.ncost[nb];
-    for (int i = 0; i < m; i++) s.head[np[i] & 127] = -1;
-    int mn = 0;
-    for (int i = 1; i < m; i++)
-        if (bits(nc[i]) < bits(nc[mn])) mn = i;
-    int v1 = 0;
-    for (int i = 0; i < m; i++) v1 += s.nnext[i] & 1;
-    int maj = (v1 > m - v1) ? 1 : 0;
-    if (s.nnext[mn] != maj) {
-        int bm = -1;
-        for (int i = 0; i < m; i++)
-            if (s.nnext[i] == maj && (bm < 0 || bits(nc[i]) < bits(nc[bm]))) bm = i;
-        if (bm >= 0 && nc[bm] < nc[mn] * 1.1) mn = bm;
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u64)v, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((u64)v >> 32), l);
+    return (long long)(((u64)hi << 32) | lo);
+}
+
+/* The reference's sequential candidate sort (src/trellis.c:582-731) stepped
+ * literally, one child after the other in index order, with every lane
+ * taking part: lane k holds entry k of the active list (child and metric
+ * bits), so the insertion point is a ballot and a shift is one lane
+ * exchange.  The path-hash chains are walked as the reference walks them,
+ * including links into the context this one was copied from and the write
+ * of the 33rd list entry onto bucket 0's head in bank 1 (act[] has 32
+ * entries and the hash heads of bank 1 follow it).  Used for the steps the
+ * parallel prune does not decide.  Writes rank[] (255 = pruned) and returns
+ * the number of survivors. */
+template <int P, int HW>
+__device__ int wide_prune(TrLds<P, HW> &s, int nb, int m, int K, const tr_ctx *snap, int *err,
+                          double cost, uint32_t path, int cnext, u64 LT, u64 N1, int lane, bool live)
+{
+    const u64 ALL = (m == 64) ? ~0ull : ((1ull << m) - 1ull);
+    const long long mycb = bits(cost);
+    if (live) s.head[path & 127u] = (int8_t)-1;
+    wsync();
+    /* first strict minimum; majority of the inherited bits (tie -> 0) and
+     * the cheapest child of the majority */
+    const int min0 = ff1(ballot(live && (LT & ALL) == 0));
+    const int v1 = __popcll(N1 & ALL);
+    const int maj = (v1 > m - v1) ? 1 : 0;
+    int mn = min0;
+    if ((int)((N1 >> min0) & 1) != maj) {
+        const u64 MJ = maj ? (N1 & ALL) : (ALL & ~N1);
+        const u64 bmm = ballot(live && ((MJ >> lane) & 1) && (LT & MJ) == 0);
+        if (bmm) {
+            const int bm = ff1(bmm);
+            if (rl_d(cost, bm) < rl_d(cost, min0) * 1.1) mn = bm;
+        }
     }
-    const int M = s.nnext[mn];
-    int n = 0;
-    for (int i = 0; i < m; i++) {
-        if (s.nnext[i] != M) continue;
-        const long long ci = bits(nc[i]);
-        if (n == K && bits(nc[s.act[n - 1]]) <= ci) continue;
-        /* lookup along the hash chain */
-        const uint32_t pi = np[i];
-        const int b = (int)(pi & 127);
+    const u64 CAND = ((N1 >> mn) & 1) ? (N1 & ALL) : (ALL & ~N1);
+
+    int n = 0;                  /* list length (uniform) */
+    int aidx = 0;               /* lane k < n: child listed at k */
+    long long acb = 0;          /* and its metric bits */
+    u64 rest = CAND;
+    while (rest) {
+        const int i = ff1(rest);
+        rest &= rest - 1;
+        const long long ci = rl_ll(mycb, i);
+        if (n == K && rl_ll(acb, n - 1) <= ci) continue;
+        const uint32_t pi = (uint32_t)__builtin_amdgcn_readlane((int)path, i);
+        const int b = (int)(pi & 127u);
+        /* lookup along the hash chain (src/trellis.c:599-619) */
         int t = -1;
-        long long tcost = 0;
-        int v = s.head[b];
+        long long tcb = 0;
+        int v = uni(s.head[b]);
         int hops = 0;
         while (v != -1) {
             if (++hops > 2 * TR_SLOTS + 2) { *err = 2; break; }   /* links only point downwards */
@@ -239,66 +296,76 @@ __device__ int literal_prune(TrLds<P, HW> &s, int nb, int m, int K,
             long long vc;
             int vl;
             if (v < TR_LINK_FOREIGN) {
-                vp = np[v];
-                vc = bits(nc[v]);
-                vl = s.link[nb][v];
+                vp = (uint32_t)uni((int)s.npath[nb][v]);
+                vc = rl_ll(bits(s.ncost[nb][v]), 0);
+                vl = uni(s.link[nb][v]);
             } else {
                 if (!snap) { *err = 2; break; }
                 const int sv = v - TR_LINK_FOREIGN;
-                vp = snap->node_path[nb][sv];
-                vc = bits(snap->node_cost[nb][sv]);
-                vl = snap->link[nb][sv];
+                vp = (uint32_t)uni((int)snap->node_path[nb][sv]);
+                vc = rl_ll(bits(snap->node_cost[nb][sv]), 0);
+                vl = uni(snap->link[nb][sv]);
                 if (vl >= TR_LINK_FOREIGN) { *err = 2; break; }
                 if (vl >= 0) vl += TR_LINK_FOREIGN;
             }
-            if (vp == pi) { t = v; tcost = vc; break; }
+            if (vp == pi) { t = v; tcb = vc; break; }
             v = vl;
         }
+        int tp = n;             /* list position of the registrant (n: not listed) */
+        bool guarded = true;
         if (t < 0) {
-            s.link[nb][i] = s.head[b];
-            s.head[b] = (int8_t)i;
-            int j;
-            for (j = n; j > 0; j--) {
-                int r = s.act[j - 1];
-                if (bits(nc[r]) <= ci) break;
-                if (j == TR_MAXK) {
-                    if (nb == 1) s.head[0] = (int8_t)r;   /* act[32] aliases bucket 0 */
-                } else {
-                    s.act[j] = (int8_t)r;
+            /* register at the head of the bucket */
+            const int h = uni(s.head[b]);
+            if (lane == 0) {
+                s.link[nb][i] = (int8_t)h;
+                s.head[b] = (int8_t)i;
+            }
+            wsync();
+            guarded = false;
+        } else {
+            if (tcb <= ci) continue;
+            if (t < TR_LINK_FOREIGN) {
+                const u64 hit = ballot(lane < n && aidx == t);
+                if (hit) tp = ff1(hit);
+            }
+            if (tp < n) {
+                /* remove the registrant's entry */
+                const int ai = __shfl_down(aidx, 1, 64);
+                const long long ab = (long long)(((u64)(unsigned)__shfl_down((int)((u64)acb >> 32), 1, 64) << 32) |
+                                                 (unsigned)__shfl_down((int)(unsigned)(u64)acb, 1, 64));
+                if (lane >= tp) {
+                    aidx = ai;
+                    acb = ab;
                 }
+                n--;
             }
-            if (j < K) s.act[j] = (int8_t)i;
-            if (n < K) n++;
-            continue;
         }
-        if (tcost <= ci) continue;
-        int tp = n;
-        if (t < TR_LINK_FOREIGN)
-            for (tp = 0; tp < n; tp++)
-                if (s.act[tp] == t) break;
-        int j;
-        if (tp == n) {
-            for (j = n; j > 0; j--) {
-                int r = s.act[j - 1];
-                if (bits(nc[r]) <= ci) break;
-                if (j < K) s.act[j] = (int8_t)r;
+        /* stable insertion behind equal metrics */
+        const int j = __popcll(ballot(lane < n && acb <= ci));
+        if (!guarded && n == TR_MAXK && nb == 1) {
+            /* the shift writes act[32], which is bank 1's bucket-0 head */
+            const int r = __builtin_amdgcn_readlane(aidx, TR_MAXK - 1);
+            if (lane == 0) s.head[0] = (int8_t)r;
+            wsync();
+        }
+        {
+            const int ai = __shfl_up(aidx, 1, 64);
+            const long long ab = (long long)(((u64)(unsigned)__shfl_up((int)((u64)acb >> 32), 1, 64) << 32) |
+                                             (unsigned)__shfl_up((int)(unsigned)(u64)acb, 1, 64));
+            if (lane > j) {
+                aidx = ai;
+                acb = ab;
+            } else if (lane == j) {
+                aidx = i;
+                acb = ci;
             }
-            if (j < K) s.act[j] = (int8_t)i;
-            if (n < K) n++;
-            continue;
         }
-        for (j = tp; j + 1 < n; j++) s.act[j] = s.act[j + 1];
-        n--;
-        for (j = n; j > 0; j--) {
-            int r = s.act[j - 1];
-            if (bits(nc[r]) <= ci) break;
-            if (j < K) s.act[j] = (int8_t)r;
-        }
-        if (j < K) s.act[j] = (int8_t)i;
-        n++;
+        if (n < K) n++;
     }
-    for (int i = 0; i < 64; i++) s.rank[i] = 255;
-    for (int k = 0; k < n; k++) s.rank[s.act[k]] = (uint8_t)k;
+    s.rank[lane] = 255;
+    wsync();
+    if (lane < n) s.rank[aidx] = (uint8_t)lane;
+    wsync();
     return n;
 }
 
@@ -487,6 +554,19 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
                 if (live) s.rank[rall] = (uint8_t)lane;
                 wsync();
                 tie = ballot(live && s.rank[rall] != lane) != 0;
+                if (tie) {
+                    /* Every rule of the sequential sort compares a child with
+                     * earlier ones as "cost <= mine" and with later ones as
+                     * "cost < mine" (first strict minimum, skip when the
+                     * full list's last is <= mine, drop a duplicate whose
+                     * registrant is <= mine, insertion behind equals), so it
+                     * follows the total order (metric, index): equal metrics
+                     * rank by index. */
+                    const u64 LE = (m <= 16) ? le_sweep<16>(keys, cost)
+                                 : (m <= 32) ? le_sweep<32>(keys, cost) : le_sweep<64>(keys, cost);
+                    LT = (LE & lowerm) | (LT & ~lowerm);
+                    tie = false;
+                }
             }
         }
         bool slow = !groups_ok || tie;
@@ -591,16 +671,9 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
         }
         if (slow) {
             slow_steps++;
-            if (live) s.nnext[lane] = (uint8_t)cnext;
-            wsync();
-            if (lane == 0) {
-                int e2 = 0;
-                s.scal[0] = literal_prune<P, HW>(s, nb, m, K, snap, &e2);
-                s.scal[1] = e2;
-            }
-            wsync();
-            nsurv = s.scal[0];
-            if (s.scal[1]) err = s.scal[1];
+            int e2 = 0;
+            nsurv = wide_prune<P, HW>(s, nb, m, K, snap, &e2, cost, path, cnext, LT, N1, lane, live);
+            if (e2) err = e2;
             myrank = live ? s.rank[lane] : 255;
         }
 

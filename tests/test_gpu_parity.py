@@ -34,6 +34,9 @@ TRELLIS_CASES = [
     (4, 8, 4, 2822400, 8000, "sine"),
     (1, 1, 1, 2822400, 4000, "sine"),
     (32, 200, 8, 22579200, 6000, "loud"),
+    (4, 2048, 8, 2822400, 7000, "pink"),      # longest history (32 words per survivor)
+    (2, 129, 4, 11289600, 9000, "pink"),      # K = 2 past the lane kernel's history limit
+    (32, 64, 32, 11289600, 6000, "loud"),     # 32-bit paths
 ]
 
 
@@ -141,3 +144,36 @@ def test_das_bit_exact(gpu, oracle, K, L, B, rate, F, seg_len, nseg, seg0, kind)
     for c in range(nch):
         bad = np.nonzero(y[c, :n] != ref[c, :n])[0]
         assert bad.size == 0, "ch%d first mismatch %d" % (c, bad[0])
+
+
+@pytest.mark.parametrize("K", [16, 32])
+def test_equal_metrics_in_parallel_prune(gpu, oracle, K):
+    """Seeded survivors share one state and metric 0, so their children tie
+    exactly for several steps.  The sequential sort orders equal metrics by
+    index; the parallel prune decides those steps in that order (below
+    K = 32 no step needs the literal prune) and matches the oracle."""
+    import torch
+    rate = 11289600
+    fi = oracle.ntf_auto(rate)
+    x = _x("pink", 16000, rate, 23)
+    o_src = oracle.ctx(fi, 8, K, 64)
+    o_src.process(x[:6000])
+    st = o_src.best_state()
+    o_dst = oracle.ctx(fi, 8, K, 64)
+    o_dst.copy_from(o_src)
+    o_dst.seed(st)
+    ref = o_dst.process(x[6000:])
+    p = T.SdmPool(fi, 8, K, 64, 0.0, 2)
+    p.process(0, x[:6000])
+    d_in = torch.from_numpy(np.ascontiguousarray(x[6000:])).cuda()
+    n = d_in.numel()
+    d_out = torch.zeros(n // 64 + 2, dtype=torch.int64, device="cuda")
+    seed = np.full((1, 10), np.nan)
+    seed[0, :] = st
+    kept, slow = p.run_jobs([1], [d_in.data_ptr()], [n], [d_out.data_ptr()], copy_src=[0], seed=seed)
+    torch.cuda.synchronize()
+    assert kept[0] == len(ref)
+    y = T.unpack_bits(d_out.cpu().numpy(), int(kept[0]))
+    assert np.array_equal(y, ref)
+    if K < 32:
+        assert slow[0] == 0
