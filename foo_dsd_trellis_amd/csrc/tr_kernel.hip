@@ -7,15 +7,15 @@
  *                 reference's scalar operation order (FP64, no contraction,
  *                 src/trellis.c:37-191) and forms the child metric;
  *   2. prune    : a wave-parallel formulation of the reference's sequential
- *                 candidate sort (src/trellis.c:621-731): strict cost-order
- *                 masks from an LDS broadcast sweep, majority/traceback
- *                 filter, then a fixed-point resolution of the early skip
- *                 and of the duplicate-path rules, and a final rank;
- *                 steps that fall outside what the parallel form decides
- *                 (equal metrics, a bucket-0 lookup after the full-list
- *                 write past act[31] at K=32, paths wider than 8 bits) are
- *                 decided by a literal sequential prune run by lane 0 over
- *                 the same LDS state;
+ *                 candidate sort (src/trellis.c:621-731): order masks by
+ *                 (metric, index) from an LDS broadcast sweep, majority/
+ *                 traceback filter, then a fixed-point resolution of the
+ *                 early skip and of the duplicate-path rules, and a final
+ *                 rank; steps that fall outside what the parallel form
+ *                 decides (a bucket-0 lookup after the full-list write past
+ *                 act[31] at K=32, paths wider than 8 bits) step the
+ *                 sequential sort literally, child by child, with the list
+ *                 spread over the lanes;
  *   3. commit   : survivors write state/metric/path/history for the next
  *                 step into the other LDS bank; the traceback bit of the
  *                 best survivor is the output (src/trellis.c:938-972).
