@@ -111,6 +111,20 @@ __device__ __forceinline__ int bit_at(const u64 *w, int64_t i) { return (int)((w
  * Outputs per boundary: ncand, peak, and per candidate pos / density /
  * max |decode| (the host applies the density ratio and selection).
  */
+/* first set bit of `w` at or after position p (limit if none before it) */
+__device__ __forceinline__ int next_set(const u64 *w, int p, int limit)
+{
+    if (p >= limit) return limit;
+    int i = p >> 6;
+    u64 x = w[i] & (~0ull << (p & 63));
+    while (!x) {
+        if (++i * 64 >= limit) return limit;
+        x = w[i];
+    }
+    const int r = i * 64 + __builtin_ctzll(x);
+    return r < limit ? r : limit;
+}
+
 extern "C" __global__ __launch_bounds__(256) void das_stitch(
     const u64 *__restrict__ bits, const int64_t *__restrict__ prev_bit,
     const int64_t *__restrict__ cur_bit, const int *__restrict__ ovl_len, int hw_lat,
@@ -118,16 +132,22 @@ extern "C" __global__ __launch_bounds__(256) void das_stitch(
     int *__restrict__ o_pos, int *__restrict__ o_dens, double *__restrict__ o_maxd)
 {
     constexpr int MAXO = 8192;   /* overlap up to 64*128 */
+    constexpr int NW = MAXO / 64;
+    constexpr int NT = 256;
     __shared__ uint8_t match[MAXO];
-    __shared__ int psum[MAXO + 1];
+    __shared__ uint16_t psum[MAXO + 1];    /* matches before position i */
+    __shared__ uint16_t mlist[MAXO];       /* mismatch positions, ascending */
+    __shared__ u64 mword[NW], qword[NW], cword[NW];
+    __shared__ int part[NT];
     __shared__ int cpos[DAS_MAXQ + 1], cden[DAS_MAXQ + 1];
-    __shared__ int ncand, peak;
+    __shared__ int ncand, peak, ppos;
+    __shared__ int rmax[NT / 64], rarg[NT / 64];
     __shared__ double qf[DAS_QTAPS];
     __shared__ double redmax[DAS_MAXQ + 1][4];
-    const int b = blockIdx.x;
+    const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int ovl = ovl_len[b];
-    if (ovl <= 0) {            /* unused boundary slot of a fixed table */
-        if (threadIdx.x == 0) {
+    if (ovl <= 0 || ovl > MAXO) {   /* unused boundary slot of a fixed table (overlaps are <= 64*128) */
+        if (tid == 0) {
             o_ncand[b] = 0;
             o_peak[b] = 0;
         }
@@ -137,42 +157,125 @@ extern "C" __global__ __launch_bounds__(256) void das_stitch(
     int hw = hw_lat;
     if (hw > ovl / 2) hw = ovl / 2;
     if (hw < 4) hw = 4;
+    const int nw = (ovl + 63) >> 6;
 
-    for (int i = threadIdx.x; i < DAS_QTAPS; i += blockDim.x) qf[i] = qfir[i];
-    for (int i = threadIdx.x; i < ovl; i += blockDim.x)
-        match[i] = (uint8_t)(bit_at(bits, pb + i) == bit_at(bits, cbit + i));
-    __syncthreads();
-    /* prefix sums of matches (serial in one lane; ovl <= 8192) */
-    if (threadIdx.x == 0) {
-        int c = 0;
-        psum[0] = 0;
-        for (int i = 0; i < ovl; i++) { c += match[i]; psum[i + 1] = c; }
+    for (int i = tid; i < DAS_QTAPS; i += NT) qf[i] = qfir[i];
+    for (int i = tid; i < ovl; i += NT) match[i] = (uint8_t)(bit_at(bits, pb + i) == bit_at(bits, cbit + i));
+    /* this stream's overlap bits, and the match bits, as words */
+    for (int wi = wv; wi < nw; wi += NT / 64) {
+        const int p = wi * 64 + lane;
+        const bool in = p < ovl;
+        const int pc = in ? p : 0;
+        const int cur = bit_at(bits, cbit + pc);
+        const u64 cb = __builtin_amdgcn_ballot_w64(in && cur);
+        const u64 mb = __builtin_amdgcn_ballot_w64(in && bit_at(bits, pb + pc) == cur);
+        if (lane == 0) {
+            cword[wi] = cb;
+            mword[wi] = mb;
+        }
     }
     __syncthreads();
-    /* density at p: matches in [max(0,p-hw), min(ovl, p+hw)) — the
+    /* prefix counts of matches: per-thread chunks, then a scan of the 256
+     * chunk sums in wave 0 */
+    const int C = (ovl + NT - 1) / NT;
+    const int c0 = tid * C, c1 = (c0 + C < ovl) ? c0 + C : ovl;
+    {
+        int cnt = 0;
+        for (int i = c0; i < c1; i++) cnt += match[i];
+        part[tid] = cnt;
+    }
+    __syncthreads();
+    if (wv == 0) {
+        const int a0 = part[4 * lane], a1 = part[4 * lane + 1], a2 = part[4 * lane + 2], a3 = part[4 * lane + 3];
+        const int tot = a0 + a1 + a2 + a3;
+        int inc = tot;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int o = __shfl_up(inc, off, 64);
+            if (lane >= off) inc += o;
+        }
+        const int ex = inc - tot;
+        part[4 * lane] = ex;
+        part[4 * lane + 1] = ex + a0;
+        part[4 * lane + 2] = ex + a0 + a1;
+        part[4 * lane + 3] = ex + a0 + a1 + a2;
+    }
+    __syncthreads();
+    {
+        int c = part[tid];
+        for (int i = c0; i < c1; i++) {
+            psum[i] = (uint16_t)c;
+            if (!match[i]) mlist[i - c] = (uint16_t)i;
+            c += match[i];
+        }
+        if (c1 == ovl && c0 < c1) psum[ovl] = (uint16_t)c;
+    }
+    __syncthreads();
+    /* density at p: matches in [max(0,p-hw), min(ovl, p+hw)) -- the
      * reference's sliding window (:2465-2482) */
     auto dm = [&](int p) {
         int lo = p - hw; if (lo < 0) lo = 0;
         int hi = p + hw; if (hi > ovl) hi = ovl;
-        return psum[hi] - psum[lo];
+        return (int)psum[hi] - (int)psum[lo];
     };
-    if (threadIdx.x == 0) {
-        int pk = 0;
-        for (int p = 0; p < ovl; p++) if (dm(p) > pk) pk = dm(p);
-        const int fl = pk * 9 / 10;
+    /* peak density and its first position */
+    {
+        int best = -1, arg = 0;
+        for (int p = tid; p < ovl; p += NT) {
+            const int d = dm(p);
+            if (d > best) { best = d; arg = p; }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const int ob = __shfl_down(best, off, 64), oa = __shfl_down(arg, off, 64);
+            if (ob > best || (ob == best && oa < arg)) { best = ob; arg = oa; }
+        }
+        if (lane == 0) { rmax[wv] = best; rarg[wv] = arg; }
+        __syncthreads();
+        if (tid == 0) {
+            int bb = rmax[0], ba = rarg[0];
+            for (int w2 = 1; w2 < NT / 64; w2++)
+                if (rmax[w2] > bb || (rmax[w2] == bb && rarg[w2] < ba)) { bb = rmax[w2]; ba = rarg[w2]; }
+            peak = bb < 0 ? 0 : bb;
+            ppos = ba;
+        }
+        __syncthreads();
+    }
+    const int pk = peak;
+    const int fl = pk * 9 / 10;
+    /* run starts that qualify: matching, density >= 90 % of the peak */
+    for (int wi = wv; wi < nw; wi += NT / 64) {
+        const int p = wi * 64 + lane;
+        const u64 qb = __builtin_amdgcn_ballot_w64(p < ovl - 3 && match[p] && dm(p) >= fl);
+        if (lane == 0) qword[wi] = qb;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        /* candidate runs in scan order (:2484-2530): from each qualifying
+         * position the run takes every following match; runs shorter than 4
+         * or closer than hw to the last candidate are passed over */
         int nq = 0;
-        for (int p = 0; p < ovl - 3 && nq < DAS_MAXQ;) {
-            if (!match[p] || dm(p) < fl) { p++; continue; }
-            int rs = p, rl = 0;
-            while (p < ovl && match[p]) { rl++; p++; }
+        int p = 0;
+        while (nq < DAS_MAXQ) {
+            p = next_set(qword, p, ovl - 3);
+            if (p >= ovl - 3) break;
+            const int rs = p;
+            /* first non-match at or after p */
+            int e = p;
+            {
+                int i = e >> 6;
+                u64 x = ~mword[i] & (~0ull << (e & 63));
+                while (!x && ++i < nw) x = ~mword[i];
+                e = x ? i * 64 + __builtin_ctzll(x) : ovl;
+                if (e > ovl) e = ovl;
+            }
+            const int rl = e - rs;
+            p = e;
             if (rl < 4) continue;
             if (nq > 0 && rs - cpos[nq - 1] < hw) continue;
             cpos[nq] = rs;
             cden[nq] = dm(rs);
             nq++;
         }
-        int pp = 0;
-        for (int p = 1; p < ovl; p++) if (dm(p) > dm(pp)) pp = p;
+        const int pp = ppos;
         int prp = pp, prl = 1;
         for (int mr = 8; mr >= 1 && prl < 4; mr >>= 1)
             for (int r = 0; r <= hw && prl < 4; r++)
@@ -196,26 +299,31 @@ extern "C" __global__ __launch_bounds__(256) void das_stitch(
         }
         if (nq == 0) { cpos[0] = 0; cden[0] = 0; nq = 1; }
         ncand = nq;
-        peak = pk;
     }
     __syncthreads();
     const int nq = ncand;
-    /* decode of (this - prev) from each candidate on: lanes over (cand, j) */
+    /* decode of (this - prev) from each candidate on: lanes over the 1024
+     * shifts; a shift sums q[k] * (+-2) over the mismatching positions of
+     * its 255-tap window from the candidate on, in ascending k (the
+     * matching ones contribute zero terms) */
     for (int ci = 0; ci < nq; ci++) {
         double mx = 0.0;
         if (nq > 1) {
             const int sp = cpos[ci];
-            for (int jj = threadIdx.x; jj < 2 * DAS_EVAL; jj += blockDim.x) {
+            for (int jj = tid; jj < 2 * DAS_EVAL; jj += NT) {
                 const int j = jj - DAS_EVAL;
+                const int base = sp + j - 127;             /* position of tap 0 */
+                int lo = base > sp ? base : sp;
+                int hi = base + DAS_QTAPS;
+                if (hi > ovl) hi = ovl;
                 double val = 0.0;
-                int k0 = sp - (sp + j - 127);   /* first k with idx >= sp */
-                if (k0 < 0) k0 = 0;
-                for (int k = k0; k < DAS_QTAPS; k++) {
-                    const int idx = sp + j - 127 + k;
-                    if (idx >= ovl) break;
-                    if (idx < 0 || match[idx]) continue;
-                    const double diff = bit_at(bits, cbit + idx) ? 2.0 : -2.0;
-                    val = val + qf[k] * diff;
+                if (lo < hi) {
+                    const int m1 = hi - (int)psum[hi];
+                    for (int mi = lo - (int)psum[lo]; mi < m1; mi++) {
+                        const int idx = mlist[mi];
+                        const double diff = ((cword[idx >> 6] >> (idx & 63)) & 1ull) ? 2.0 : -2.0;
+                        val = val + qf[idx - base] * diff;
+                    }
                 }
                 const double ad = fabs(val);
                 if (ad > mx) mx = ad;
@@ -226,18 +334,18 @@ extern "C" __global__ __launch_bounds__(256) void das_stitch(
             const double o = __shfl_down(mx, off, 64);
             if (o > mx) mx = o;
         }
-        if ((threadIdx.x & 63) == 0) redmax[ci][threadIdx.x >> 6] = mx;
+        if (lane == 0) redmax[ci][wv] = mx;
     }
     __syncthreads();
-    if (threadIdx.x < nq) {
-        const int ci = threadIdx.x;
+    if (tid < nq) {
+        const int ci = tid;
         double mx = 0.0;
         for (int w = 0; w < 4; w++) if (redmax[ci][w] > mx) mx = redmax[ci][w];
         o_pos[b * (DAS_MAXQ + 1) + ci] = cpos[ci];
         o_dens[b * (DAS_MAXQ + 1) + ci] = cden[ci];
         o_maxd[b * (DAS_MAXQ + 1) + ci] = mx;
     }
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         o_ncand[b] = nq;
         o_peak[b] = peak;
     }
