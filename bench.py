@@ -384,10 +384,18 @@ def main():
     D = das.Das(fi, B, K, L, NCH, seg_len=args.seg_len, seg0_first=args.seg0_first, reposition=True)
     D.timing(True)
 
-    def step():
+    fir_ev = []
+
+    def step(timed=False):
         chain.reset()
         D.reset()
+        if timed:
+            ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            ev[0].record()
         chain.process_dev(d_x.data_ptr(), n_in, n_in, d_f.data_ptr(), F)
+        if timed:
+            ev[1].record()
+            fir_ev.append(ev)
         return D.run_dev(d_f.data_ptr(), F, F, d_bits.data_ptr(), words)
 
     for _ in range(args.warmup):
@@ -397,9 +405,19 @@ def main():
     t0 = time.perf_counter()
     n_out = 0
     for _ in range(args.steps):
-        n_out = step()
+        n_out = step(timed=True)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    # FIR chain (8 half-band stages): algorithmic bytes = first-stage float
+    # input + every stage's double output + the double inputs of stages 2..8
+    fir_ms = sum(a.elapsed_time(b) for a, b in fir_ev) / max(1, len(fir_ev))
+    nst = int(round(np.log2(DSD256 // FS_IN)))
+    fir_bytes = NCH * n_in * (4 + 16 * (2 ** nst - 1) + 8 * (2 ** nst - 2))
+    fir_roof = {"bound": "hbm", "ms": round(fir_ms, 3), "bytes": int(fir_bytes),
+                "achieved_GBps": round(fir_bytes / (fir_ms / 1e3) / 1e9, 1) if fir_ms > 0 else None,
+                "peak_GBps": 8000.0,
+                "frac": round(fir_bytes / (fir_ms / 1e3) / 8e12, 4) if fir_ms > 0 else None,
+                "stages": nst, "note": "whole chain per step on the default stream, HIP events"}
     k_ms, k_launches, k_samples = D.timing_read(clear=True)
     ms_step = (t1 - t0) * 1000.0 / args.steps
     total_out = NCH * n_out
@@ -458,6 +476,7 @@ def main():
                    "seed_mode": "seg0-first" if args.seg0_first else "all-parallel",
                    "literal_prune_steps": slow, "out_samples_per_channel": n_out},
         "roofline": roof,
+        "fir_roofline": fir_roof,
         "plain_gpu": plain,
         "cpu_baseline": cpu,
         "diff_loop_filter": lf,

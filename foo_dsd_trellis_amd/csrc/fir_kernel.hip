@@ -11,8 +11,9 @@
  * positions of one channel (blockIdx.y).  The input tile plus its Q-1
  * sample history is staged in LDS with coalesced loads; each thread keeps a
  * sliding window of R+1 inputs in registers and walks q, so every LDS value
- * is read once per q-step and reused for 2R outputs.  Outputs of a thread
- * are 2R consecutive doubles, so a wave stores one contiguous span.
+ * is read once per q-step and reused for 2R outputs.  The 2R outputs of a
+ * thread are consecutive, so they go through LDS once more and leave as
+ * contiguous 16-byte lanes (a wave stores 1 KB per instruction).
  * The last stage multiplies by the engine gain (src/engine.c:872-883).
  */
 #include <hip/hip_runtime.h>
@@ -34,6 +35,7 @@ __global__ __launch_bounds__(FIR_THREADS) void fir_up2_kernel(
     constexpr int Q1 = T / 2;         /* odd-phase taps */
     constexpr int H = Q0 - 1;         /* history samples needed */
     __shared__ double tile[FIR_TILE + H];
+    __shared__ double2 ost[FIR_TILE];
 
     const int ch = blockIdx.y;
     const TIn *xc = x + (int64_t)ch * x_stride;
@@ -78,21 +80,24 @@ __global__ __launch_bounds__(FIR_THREADS) void fir_up2_kernel(
             w[0] = tile[base - (q + 1)];
         }
     }
-    const int64_t mt = m0 + (int64_t)FIR_R * threadIdx.x;
 #pragma unroll
     for (int r = 0; r < FIR_R; r++) {
-        const int64_t m = mt + r;
-        if (m < n) {
-            double e = acc0[r], o = acc1[r];
-            if (apply_gain) {
-                e = e * gain;
-                o = o * gain;
-            }
-            double2 v;
-            v.x = e;
-            v.y = o;
-            *reinterpret_cast<double2 *>(yc + 2 * m) = v;
+        double e = acc0[r], o = acc1[r];
+        if (apply_gain) {
+            e = e * gain;
+            o = o * gain;
         }
+        double2 v;
+        v.x = e;
+        v.y = o;
+        ost[FIR_R * threadIdx.x + r] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < FIR_R; r++) {
+        const int i = r * FIR_THREADS + threadIdx.x;
+        const int64_t m = m0 + i;
+        if (m < n) *reinterpret_cast<double2 *>(yc + 2 * m) = ost[i];
     }
 }
 
