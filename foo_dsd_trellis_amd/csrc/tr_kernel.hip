@@ -207,13 +207,10 @@ struct TrLds {
     uint32_t npath[2][64];
     uint32_t khi[64];              /* high words of this step's child metrics */
     u64 gtab[256 + 64];            /* + one scratch entry per lane */
-    u64 btab[128 + 64];
     int8_t link[2][64];
     uint8_t rank[64];
     int8_t head[128];
-    int8_t act[TR_MAXK + 1];
     int8_t src[TR_MAXK];           /* lane holding the survivor of each rank */
-    int32_t scal[4];
 };
 
 /* value of another lane's register (all lanes active) */
@@ -421,7 +418,6 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
             s.ncost[b][lane] = c0->node_cost[b][lane];
         }
         for (int i = lane; i < 256 + 64; i += 64) s.gtab[i] = 0;
-        for (int i = lane; i < 128 + 64; i += 64) s.btab[i] = 0;
     }
     wsync();
 
@@ -532,28 +528,28 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
          * high words give equal strict ranks, and only then the full 64-bit
          * sweep runs. */
         u64 LT;
-        int rall;
-        bool tie;
+        bool tie = false;
+        /* equal keys: two children with one strict rank */
+        auto has_tie = [&](u64 lt) {
+            const int r = __popcll(lt & ALL);
+            if (live) s.rank[r] = (uint8_t)lane;
+            wsync();
+            const bool t = ballot(live && s.rank[r] != lane) != 0;
+            wsync();
+            return t;
+        };
         {
             const bool neg = ballot(live && (chi >> 31)) != 0;
             if (!neg) {
                 LT = (m <= 16) ? lt_sweep_hi<16>(s.khi, chi)
                    : (m <= 32) ? lt_sweep_hi<32>(s.khi, chi) : lt_sweep_hi<64>(s.khi, chi);
-                rall = __popcll(LT & ALL);
-                if (live) s.rank[rall] = (uint8_t)lane;
-                wsync();
-                tie = ballot(live && s.rank[rall] != lane) != 0;
-                wsync();
+                tie = has_tie(LT);
             }
             if (neg || tie) {
                 const double *keys = s.ncost[nb];
                 LT = (m <= 16) ? lt_sweep<16>(keys, cost)
                    : (m <= 32) ? lt_sweep<32>(keys, cost) : lt_sweep<64>(keys, cost);
-                /* equal metrics: strict ranks must be distinct */
-                rall = __popcll(LT & ALL);
-                if (live) s.rank[rall] = (uint8_t)lane;
-                wsync();
-                tie = ballot(live && s.rank[rall] != lane) != 0;
+                tie = has_tie(LT);
                 if (tie) {
                     /* Every rule of the sequential sort compares a child with
                      * earlier ones as "cost <= mine" and with later ones as
@@ -595,6 +591,8 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
             atomicOr(&s.gtab[gi], 1ull << lane);
             wsync();
             const u64 G = s.gtab[gi] & (cand ? ~0ull : 0ull);
+            /* the group of the other path of this hash bucket (path ^ 128) */
+            const u64 G2 = cand ? s.gtab[path ^ 128u] : 0ull;
             wsync();
             s.gtab[gi] = vzero64();
 
@@ -653,15 +651,12 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
                 }
             }
             if (!slow) {
-                /* hash links of the new registrations: previous registrant
-                 * of the same bucket, in index order */
-                const int bi = reg ? (int)(path & 127u) : 128 + lane;
-                atomicOr(&s.btab[bi], 1ull << lane);
-                wsync();
-                const u64 bm = s.btab[bi] & lowerm;
-                if (reg) s.link[nb][lane] = bm ? (int8_t)(63 - __builtin_clzll(bm)) : (int8_t)-1;
-                wsync();
-                s.btab[bi] = vzero64();
+                /* hash links of the new registrations: the previous
+                 * registrant of the same bucket in index order.  With paths
+                 * below 256 a bucket holds at most two registrants, one for
+                 * each of the paths p and p ^ 128 */
+                const u64 tw = REG & G2;
+                if (reg) s.link[nb][lane] = (tw & lowerm) ? (int8_t)ff1(tw) : (int8_t)-1;
                 const u64 F = CAND & ~DS & ~KM;
                 const int rk = __popcll(LT & F);
                 myrank = (((F >> lane) & 1) && rk < K) ? rk : 255;
