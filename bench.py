@@ -260,8 +260,9 @@ def next_rows(d_bits, n_bits, seconds):
     PreCorr (greedy SDM, one sequential recurrence per channel) over 6
     channels of DSD256-rate input; the same-rate DSD256 re-encode of the
     headline's own output bits (the engine's boxcar of 64 taps and path
-    settings CLANS-6, K=2, L=16, B=4, DAS with 4,096-sample segments),
-    streamed and offline; DoP
+    settings CLANS-6, K=2, L=16, B=4), streamed in 2^20-sample chunks with
+    2,048-sample segments (more segments: lower chunk latency) and offline
+    in one 2^24-sample chunk with 4,096-sample segments; DoP
     packing and unpacking of all of those bits (HBM-bound byte work)."""
     import torch
     from foo_dsd_trellis_amd import dop, precorr, signals, stream, trellis
@@ -287,14 +288,14 @@ def next_rows(d_bits, n_bits, seconds):
     # of 2^20 samples (first `seconds` per channel), and converted offline in
     # one chunk of 2^24 samples (1.49 s); K=2 runs one lane per segment job
     fi6 = trellis.ntf_lookup(4, DSD256)            # clans-6 (the path table's DSD256 same-rate filter)
-    for name, cf, m in (("dsd_same_rate_reencode", 1 << 20, min(n_bits, n)),
-                        ("dsd_same_rate_reencode_offline", 1 << 24, min(n_bits, 1 << 24))):
+    for name, cf, m, seg in (("dsd_same_rate_reencode", 1 << 20, min(n_bits, n), 2048),
+                             ("dsd_same_rate_reencode_offline", 1 << 24, min(n_bits, 1 << 24), 4096)):
         m = max(cf, m // cf * cf)
         if m > n_bits:
             continue
         words = m // 64
         src = d_bits[:, :words].contiguous()
-        S = stream.Stream(DSD256, DSD256, 2, cf, ntf_index=fi6, K=2, L=16, depth=4, seg_len=4096,
+        S = stream.Stream(DSD256, DSD256, 2, cf, ntf_index=fi6, K=2, L=16, depth=4, seg_len=seg,
                           gain=float(np.float32(0.708)), input=stream.IN_DSD)
         S.process_dsd(src[:, :cf // 64], cf)
         S.reset()
@@ -307,7 +308,7 @@ def next_rows(d_bits, n_bits, seconds):
         dt = time.perf_counter() - t0
         res[name] = {"value": round(2 * tot / dt / 1e6, 3), "unit": "Msamples/s",
                      "x_realtime": round(m / DSD256 / dt, 3), "channels": 2, "chunk": cf,
-                     "config": "DSD256 -> DSD256, boxcar 64, CLANS-6 K=2 L=16 B=4, DAS 4096-sample segments"}
+                     "config": "DSD256 -> DSD256, boxcar 64, CLANS-6 K=2 L=16 B=4, DAS %d-sample segments" % seg}
         del S, src
     # DoP pack (i24, interleaved) and unpack of the whole headline output
     words = (n_bits // 64) // 1024 * 1024

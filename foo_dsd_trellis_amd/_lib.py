@@ -47,6 +47,10 @@ _SIGS = {
     "dsdhip_das_fma_o8": ([_vp, C.c_int], C.c_int),
     "dsdhip_trellis_wave_only": ([_vp, C.c_int], C.c_int),
     "dsdhip_das_wave_only": ([_vp, C.c_int], C.c_int),
+    "gpu_hip_fir_lowpass_setup": ([_vp, _fp, C.c_int], C.c_int),
+    "gpu_hip_fir_lowpass_f64": ([_vp, _fp, _dp, C.c_size_t, C.c_double], C.c_int),
+    "gpu_hip_fir_lowpass": ([_vp, _fp, _fp, C.c_size_t, C.c_float], C.c_int),
+    "gpu_hip_boxcar_smooth": ([_vp, _fp, _fp, C.c_size_t, C.c_int, C.c_float], C.c_int),
     "gpu_hip_trellis_fma_o8": ([_vp, C.c_int], C.c_int),
     "dsdhip_fir_plan": ([C.c_uint, C.c_uint, _ip], C.c_int),
     "dsdhip_halfband_taps": ([_dp, C.c_int], None),

@@ -151,6 +151,13 @@ extern "C" __global__ void cond_ramp_k(double *__restrict__ x, int64_t stride, i
     *p = *p * (gs + step * (double)i);
 }
 
+/* y[i] = (float)x[i] (round to nearest) */
+extern "C" __global__ void cond_narrow_k(const double *__restrict__ x, float *__restrict__ y, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = (float)x[i];
+}
+
 /* soft clip above 0.95 with a tanh knee */
 extern "C" __global__ void cond_clip_k(double *__restrict__ x, int64_t stride, int nch, int64_t n)
 {

@@ -19,6 +19,7 @@ __global__ void cond_clip_k(double *, int64_t, int, int64_t);
 __global__ void cond_lowpass_bits_k(const unsigned long long *, int64_t, int, int64_t, const unsigned long long *,
                                     int64_t, const double *, int, double, double, double *, int64_t);
 __global__ void cond_bits_hist2_k(const unsigned long long *, int64_t, int, int64_t, unsigned long long *);
+__global__ void cond_narrow_k(const double *, float *, int64_t);
 }
 
 #define LP_NTAPS 127
@@ -26,7 +27,8 @@ __global__ void cond_bits_hist2_k(const unsigned long long *, int64_t, int, int6
 struct dsdhip_cond {
     int nch, taps;
     int lp;                        /* lowpass designed */
-    double *d_lp;                  /* [127] */
+    int lp_ntaps;                  /* its tap count (<= 128) */
+    double *d_lp;                  /* [lp_ntaps] */
     unsigned long long *d_hist;    /* [nch] boxcar bit history */
     unsigned long long *d_hist2;   /* [nch][2] lowpass bit history */
     double *d_pcm;                 /* [nch][taps + 1] PCM boxcar state */
@@ -103,11 +105,29 @@ dsdhip_cond_t *dsdhip_cond_create(int nch, int boxcar_taps, unsigned lowpass_rat
              hipMalloc(&c->d_lp, sizeof(h)) == hipSuccess &&
              hipMemcpy(c->d_lp, h, sizeof(h), hipMemcpyHostToDevice) == hipSuccess;
         c->lp = ok ? 1 : 0;
+        c->lp_ntaps = LP_NTAPS;
     }
     if (!ok || dsdhip_cond_reset(c) != 0) {
         dsdhip_cond_destroy(c);
         return nullptr;
     }
+    return c;
+}
+
+/* The same with the caller's lowpass taps (1..128; the delay line holds 128
+ * samples), as the GPU seam's gpu_fir_lowpass_setup hands them over. */
+dsdhip_cond_t *dsdhip_cond_create_taps(int nch, int boxcar_taps, const double *h, int ntaps)
+{
+    if (!h || ntaps < 1 || ntaps > 128) return nullptr;
+    dsdhip_cond_t *c = dsdhip_cond_create(nch, boxcar_taps, 0);
+    if (!c) return nullptr;
+    if (hipMalloc(&c->d_lp, sizeof(double) * (size_t)ntaps) != hipSuccess ||
+        hipMemcpy(c->d_lp, h, sizeof(double) * (size_t)ntaps, hipMemcpyHostToDevice) != hipSuccess) {
+        dsdhip_cond_destroy(c);
+        return nullptr;
+    }
+    c->lp = 1;
+    c->lp_ntaps = ntaps;
     return c;
 }
 
@@ -159,11 +179,21 @@ int dsdhip_cond_dsd_lowpass_dev(dsdhip_cond_t *c, const unsigned long long *d_bi
     if (words_stride < ((int64_t)n + 63) / 64 || out_stride < (int64_t)n) return -1;
     const double gstep = (n > 1) ? (ge - gs) / (double)(n - 1) : 0.0;
     hipLaunchKernelGGL(cond_lowpass_bits_k, dim3(nblk((int64_t)n), (unsigned)c->nch), dim3(256), 0, c->stream,
-                       d_bits, words_stride, c->nch, (int64_t)n, c->d_hist2, c->done_lp, c->d_lp, LP_NTAPS, gs, gstep,
+                       d_bits, words_stride, c->nch, (int64_t)n, c->d_hist2, c->done_lp, c->d_lp, c->lp_ntaps, gs, gstep,
                        d_out, out_stride);
     hipLaunchKernelGGL(cond_bits_hist2_k, dim3((unsigned)((c->nch + 63) / 64)), dim3(64), 0, c->stream, d_bits, words_stride, c->nch, (int64_t)n,
                        c->d_hist2);
     c->done_lp += (int64_t)n;
+    return launched();
+}
+
+/* f64 -> f32 on the device (the float outputs of the GPU seam) */
+int dsdhip_f64_to_f32_dev(const double *d_x, float *d_y, size_t n, void *stream)
+{
+    if (!d_x || !d_y) return -1;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(cond_narrow_k, dim3(nblk((int64_t)n)), dim3(256), 0, (hipStream_t)stream, d_x, d_y,
+                       (int64_t)n);
     return launched();
 }
 

@@ -26,6 +26,8 @@ void dsdhip_make_qfir(double *q);
 /* Device scratch of the FIR chain for n input samples per channel, so later
  * dsdhip_fir_process_dev calls of at most n samples allocate nothing. */
 int dsdhip_fir_reserve(dsdhip_fir_t *f, size_t n);
+/* f64 -> f32 copy on the device (round to nearest). */
+int dsdhip_f64_to_f32_dev(const double *d_x, float *d_y, size_t n, void *stream);
 
 #ifdef __cplusplus
 }

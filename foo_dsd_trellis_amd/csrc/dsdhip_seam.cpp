@@ -10,11 +10,17 @@
 #include "dsdhip_internal.h"
 
 #define GPU_MIN_SAMPLES 8192   /* include/gpu_compute.h:40 */
+#define SEAM_MAX_CH 8          /* channel slots of the per-call lowpass / boxcar */
 
 struct gpu_hip_context {
     int backend;   /* GPU_BACKEND_HIP, read by the dispatcher (src/gpu_compute.c:153-157) */
-    dsdhip_fir_t *fir;
+    dsdhip_fir_t *fir;             /* gpu_hip_fir_batch: all channels at once */
     int fir_nch;
+    /* gpu_hip_fir_chain(_f64): one channel per call, in channel order; each
+     * channel slot keeps its own delay lines (the slot advances per call,
+     * gpu_hip_reset_chunk rewinds it) */
+    dsdhip_fir_t *fir1[SEAM_MAX_CH];
+    int fir_ch;
     unsigned fs_in, fs_out;
     double fir_gain;
     /* trellis setup */
@@ -31,6 +37,16 @@ struct gpu_hip_context {
     /* uniform-taps FIR setup (gpu_hip_fir_setup) */
     double taps[127];
     int ntaps, nstages;
+    /* same-rate lowpass and boxcar of DSD input: one history per channel
+     * slot, the slot advancing per call and reset by gpu_hip_reset_chunk
+     * (the seam processes one channel per call, in channel order) */
+    double lp_taps[128];
+    int lp_ntaps, lp_ch, box_taps, box_ch;
+    dsdhip_cond_t *lp[SEAM_MAX_CH], *box[SEAM_MAX_CH];
+    float *d_sin;
+    unsigned long long *d_sbits;
+    double *d_sout;
+    size_t scap;
 };
 
 extern "C" {
@@ -52,13 +68,123 @@ void gpu_hip_destroy(void *ctx)
 {
     gpu_hip_context *c = (gpu_hip_context *)ctx;
     if (!c) return;
+    for (int i = 0; i < SEAM_MAX_CH; i++) {
+        dsdhip_cond_destroy(c->lp[i]);
+        dsdhip_cond_destroy(c->box[i]);
+        dsdhip_fir_destroy(c->fir1[i]);
+    }
+    if (c->d_sin) hipFree(c->d_sin);
+    if (c->d_sbits) hipFree(c->d_sbits);
+    if (c->d_sout) hipFree(c->d_sout);
     dsdhip_fir_destroy(c->fir);
     dsdhip_trellis_destroy(c->tr);
     dsdhip_das_destroy(c->das);
     free(c);
 }
 
-void gpu_hip_reset_chunk(void *ctx) { (void)ctx; /* no per-chunk counters on this backend */ }
+void gpu_hip_reset_chunk(void *ctx)
+{
+    gpu_hip_context *c = (gpu_hip_context *)ctx;
+    if (c) c->fir_ch = c->lp_ch = c->box_ch = 0;
+}
+
+/* device staging for one channel of count samples: +-1.0 floats, their
+ * packed bits, the f64 result */
+static int seam_stage(gpu_hip_context *c, size_t count)
+{
+    if (c->scap >= count) return 0;
+    if (c->d_sin) hipFree(c->d_sin);
+    if (c->d_sbits) hipFree(c->d_sbits);
+    if (c->d_sout) hipFree(c->d_sout);
+    c->d_sin = NULL;
+    c->d_sbits = NULL;
+    c->d_sout = NULL;
+    c->scap = 0;
+    if (hipMalloc(&c->d_sin, sizeof(float) * count) != hipSuccess ||
+        hipMalloc(&c->d_sbits, sizeof(unsigned long long) * ((count + 63) / 64)) != hipSuccess ||
+        hipMalloc(&c->d_sout, sizeof(double) * count) != hipSuccess)
+        return -1;
+    c->scap = count;
+    return 0;
+}
+
+/* replaces gpu_cuda_fir_lowpass_setup (include/gpu_compute.h:92) */
+int gpu_hip_fir_lowpass_setup(void *ctx, const float *taps, int ntaps)
+{
+    gpu_hip_context *c = (gpu_hip_context *)ctx;
+    if (!c || !taps || ntaps < 1 || ntaps > 128) return -1;
+    for (int i = 0; i < ntaps; i++) c->lp_taps[i] = (double)taps[i];
+    c->lp_ntaps = ntaps;
+    for (int i = 0; i < SEAM_MAX_CH; i++) {
+        dsdhip_cond_destroy(c->lp[i]);
+        c->lp[i] = NULL;
+    }
+    c->lp_ch = 0;
+    return 0;
+}
+
+/* One channel of +-1.0 DSD through the lowpass (history per channel slot),
+ * times gain, to f64 on the device. */
+static int seam_lowpass(gpu_hip_context *c, const float *in, size_t count, double gain)
+{
+    if (!c || !in || c->lp_ntaps < 1 || count < GPU_MIN_SAMPLES) return -1;
+    const int ch = c->lp_ch % SEAM_MAX_CH;
+    if (!c->lp[ch] && !(c->lp[ch] = dsdhip_cond_create_taps(1, 1, c->lp_taps, c->lp_ntaps))) return -1;
+    if (seam_stage(c, count) != 0) return -1;
+    c->lp_ch++;
+    const int64_t words = (int64_t)(count + 63) / 64;
+    if (hipMemcpy(c->d_sin, in, sizeof(float) * count, hipMemcpyHostToDevice) != hipSuccess) return -1;
+    if (dsdhip_f32_to_bits_dev(c->d_sin, (int64_t)count, 1, count, c->d_sbits, words, NULL) != 0) return -1;
+    return dsdhip_cond_dsd_lowpass_dev(c->lp[ch], c->d_sbits, words, count, gain, gain, c->d_sout,
+                                       (int64_t)count);
+}
+
+/* replaces gpu_cuda_fir_lowpass_f64 (include/gpu_compute.h:102-103) */
+int gpu_hip_fir_lowpass_f64(void *ctx, const float *in, double *out, size_t count, double gain)
+{
+    gpu_hip_context *c = (gpu_hip_context *)ctx;
+    if (!out || seam_lowpass(c, in, count, gain) != 0) return -1;
+    return hipMemcpy(out, c->d_sout, sizeof(double) * count, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -1;
+}
+
+/* replaces gpu_cuda_fir_lowpass (include/gpu_compute.h:97-98): the f64
+ * result narrowed to float */
+int gpu_hip_fir_lowpass(void *ctx, const float *in, float *out, size_t count, float gain)
+{
+    gpu_hip_context *c = (gpu_hip_context *)ctx;
+    if (!out || seam_lowpass(c, in, count, (double)gain) != 0) return -1;
+    if (dsdhip_f64_to_f32_dev(c->d_sout, c->d_sin, count, NULL) != 0) return -1;
+    return hipMemcpy(out, c->d_sin, sizeof(float) * count, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -1;
+}
+
+/* replaces gpu_cuda_boxcar_smooth (include/gpu_compute.h:112-113): one
+ * channel of +-1.0 DSD, boxcar of taps samples (history per channel slot)
+ * times gain, narrowed to float */
+int gpu_hip_boxcar_smooth(void *ctx, const float *in, float *out, size_t count, int taps, float gain)
+{
+    gpu_hip_context *c = (gpu_hip_context *)ctx;
+    if (!c || !in || !out || taps < 1 || taps > 64 || count < GPU_MIN_SAMPLES) return -1;
+    if (taps != c->box_taps) {
+        for (int i = 0; i < SEAM_MAX_CH; i++) {
+            dsdhip_cond_destroy(c->box[i]);
+            c->box[i] = NULL;
+        }
+        c->box_taps = taps;
+        c->box_ch = 0;
+    }
+    const int ch = c->box_ch % SEAM_MAX_CH;
+    if (!c->box[ch] && !(c->box[ch] = dsdhip_cond_create(1, taps, 0))) return -1;
+    if (seam_stage(c, count) != 0) return -1;
+    c->box_ch++;
+    const int64_t words = (int64_t)(count + 63) / 64;
+    if (hipMemcpy(c->d_sin, in, sizeof(float) * count, hipMemcpyHostToDevice) != hipSuccess) return -1;
+    if (dsdhip_f32_to_bits_dev(c->d_sin, (int64_t)count, 1, count, c->d_sbits, words, NULL) != 0) return -1;
+    if (dsdhip_cond_dsd_boxcar_dev(c->box[ch], c->d_sbits, words, count, (double)gain, (double)gain, c->d_sout,
+                                   (int64_t)count) != 0)
+        return -1;
+    if (dsdhip_f64_to_f32_dev(c->d_sout, c->d_sin, count, NULL) != 0) return -1;
+    return hipMemcpy(out, c->d_sin, sizeof(float) * count, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -1;
+}
 
 int gpu_hip_fir_setup(void *ctx, const float *taps, int ntaps, int num_stages, bool upsample)
 {
@@ -74,6 +200,11 @@ int gpu_hip_fir_setup(void *ctx, const float *taps, int ntaps, int num_stages, b
     c->nstages = num_stages;
     dsdhip_fir_destroy(c->fir);
     c->fir = NULL;
+    for (int i = 0; i < SEAM_MAX_CH; i++) {
+        dsdhip_fir_destroy(c->fir1[i]);
+        c->fir1[i] = NULL;
+    }
+    c->fir_ch = 0;
     c->fs_in = c->fs_out = 0;
     c->fir_gain = 1.0;
     return 0;
@@ -87,6 +218,11 @@ int gpu_hip_fir_setup_rates(void *ctx, unsigned fs_in, unsigned fs_out, double g
     if (dsdhip_fir_plan(fs_in, fs_out, nt) < 1) return -1;
     dsdhip_fir_destroy(c->fir);
     c->fir = NULL;
+    for (int i = 0; i < SEAM_MAX_CH; i++) {
+        dsdhip_fir_destroy(c->fir1[i]);
+        c->fir1[i] = NULL;
+    }
+    c->fir_ch = 0;
     c->fs_in = fs_in;
     c->fs_out = fs_out;
     c->fir_gain = gain;
@@ -94,24 +230,39 @@ int gpu_hip_fir_setup_rates(void *ctx, unsigned fs_in, unsigned fs_out, double g
     return 0;
 }
 
+static dsdhip_fir_t *make_fir(gpu_hip_context *c, int nch)
+{
+    if (c->fs_in) return dsdhip_fir_create(c->fs_in, c->fs_out, c->fir_gain, nch);
+    if (c->ntaps) return dsdhip_fir_create_taps(c->taps, c->ntaps, c->nstages, 1.0, nch);
+    return NULL;
+}
+
 static int ensure_fir(gpu_hip_context *c, int nch)
 {
     if (c->fir && c->fir_nch == nch) return 0;
     dsdhip_fir_destroy(c->fir);
-    if (c->fs_in) c->fir = dsdhip_fir_create(c->fs_in, c->fs_out, c->fir_gain, nch);
-    else if (c->ntaps) c->fir = dsdhip_fir_create_taps(c->taps, c->ntaps, c->nstages, 1.0, nch);
-    else c->fir = NULL;
+    c->fir = make_fir(c, nch);
     c->fir_nch = nch;
     return c->fir ? 0 : -1;
+}
+
+/* the FIR chain of this call's channel slot */
+static dsdhip_fir_t *channel_fir(gpu_hip_context *c)
+{
+    const int ch = c->fir_ch % SEAM_MAX_CH;
+    if (!c->fir1[ch] && !(c->fir1[ch] = make_fir(c, 1))) return NULL;
+    c->fir_ch++;
+    return c->fir1[ch];
 }
 
 int gpu_hip_fir_chain_f64(void *ctx, const float *in, double *out, size_t in_count, size_t *out_count)
 {
     gpu_hip_context *c = (gpu_hip_context *)ctx;
     if (!c || !in || !out || in_count < GPU_MIN_SAMPLES) return -1;
-    if (ensure_fir(c, 1) != 0) return -1;
-    if (dsdhip_fir_process(c->fir, in, in_count, out) != 0) return -1;
-    if (out_count) *out_count = in_count << dsdhip_fir_num_stages(c->fir);
+    dsdhip_fir_t *f = channel_fir(c);
+    if (!f) return -1;
+    if (dsdhip_fir_process(f, in, in_count, out) != 0) return -1;
+    if (out_count) *out_count = in_count << dsdhip_fir_num_stages(f);
     return 0;
 }
 
@@ -119,11 +270,12 @@ int gpu_hip_fir_chain(void *ctx, const float *in, float *out, size_t in_count, s
 {
     gpu_hip_context *c = (gpu_hip_context *)ctx;
     if (!c || !in || !out || in_count < GPU_MIN_SAMPLES) return -1;
-    if (ensure_fir(c, 1) != 0) return -1;
-    const size_t n = in_count << dsdhip_fir_num_stages(c->fir);
+    dsdhip_fir_t *f = channel_fir(c);
+    if (!f) return -1;
+    const size_t n = in_count << dsdhip_fir_num_stages(f);
     double *tmp = (double *)malloc(sizeof(double) * n);
     if (!tmp) return -1;
-    int rc = dsdhip_fir_process(c->fir, in, in_count, tmp);
+    int rc = dsdhip_fir_process(f, in, in_count, tmp);
     if (rc == 0) {
         for (size_t i = 0; i < n; i++) out[i] = (float)tmp[i];
         if (out_count) *out_count = n;

@@ -59,9 +59,14 @@ int gpu_hip_fir_setup(void *ctx, const float *taps, int ntaps, int num_stages, b
  * first stage towards DSD512), with the engine's output gain
  * (src/engine.c:872-883).  Extension of the seam. */
 int gpu_hip_fir_setup_rates(void *ctx, unsigned fs_in, unsigned fs_out, double gain);
-/* replaces gpu_cuda_fir_chain (include/gpu_compute.h:242-243) */
+/* replaces gpu_cuda_fir_chain (include/gpu_compute.h:242-243): one channel
+ * per call, in channel order, as the plugin calls it (src/dsp_plugin.c:
+ * 1490-1530); each channel slot keeps its own stage delay lines, the slot
+ * advancing per call and rewound by gpu_hip_reset_chunk, so every channel
+ * continues exactly like its own fir_chain_t across chunks. */
 int gpu_hip_fir_chain(void *ctx, const float *in, float *out, size_t in_count, size_t *out_count);
-/* replaces gpu_cuda_fir_chain_f64 (include/gpu_compute.h:244-245) */
+/* replaces gpu_cuda_fir_chain_f64 (include/gpu_compute.h:244-245), per
+ * channel slot like gpu_hip_fir_chain */
 int gpu_hip_fir_chain_f64(void *ctx, const float *in, double *out, size_t in_count,
                           size_t *out_count);
 /* replaces gpu_cuda_fir_batch (include/gpu_compute.h:248-250) */
@@ -85,6 +90,22 @@ int gpu_hip_trellis_das(void *ctx, const double *in, float *out, size_t count, i
  * at :904-927 for the first 4*floor(n/4) candidates of every step) instead
  * of the scalar order.  Extension of the seam; off by default. */
 int gpu_hip_trellis_fma_o8(void *ctx, int on);
+/* replaces gpu_cuda_fir_lowpass_setup (include/gpu_compute.h:92): taps of
+ * the same-rate lowpass (1..128), widened to double */
+int gpu_hip_fir_lowpass_setup(void *ctx, const float *taps, int ntaps);
+/* replace gpu_cuda_fir_lowpass / _f64 (include/gpu_compute.h:97-103): one
+ * channel of +-1.0 DSD per call, in channel order (the channel slot advances
+ * per call and gpu_hip_reset_chunk rewinds it); y = (sum_k taps[k] s[i-k]) *
+ * gain in ascending k over the channel's persistent delay line, zero before
+ * the stream's start as in the engine's lowpass (src/fir.c:795-873).
+ * count >= GPU_MIN_SAMPLES. */
+int gpu_hip_fir_lowpass_f64(void *ctx, const float *in, double *out, size_t count, double gain);
+int gpu_hip_fir_lowpass(void *ctx, const float *in, float *out, size_t count, float gain);
+/* replaces gpu_cuda_boxcar_smooth (include/gpu_compute.h:112-113): the
+ * engine's boxcar of taps (1..64) +-1.0 samples times gain (exact window
+ * sums, src/engine.c:533-546), per channel slot like the lowpass, narrowed
+ * to float */
+int gpu_hip_boxcar_smooth(void *ctx, const float *in, float *out, size_t count, int taps, float gain);
 /* DAS plan parameters for gpu_hip_trellis_das (segment length in samples,
  * 0 = reference auto count; seed mode 1 = seg0-first, 0 = all segments
  * seeded from the persistent state; reposition quiet boundaries). */
@@ -454,6 +475,8 @@ int dsdhip_lowpass_taps(unsigned rate, double *h);
 /* nch channels, boxcar_taps (1..64; the engine's taps, src/engine.c:131-
  * 147), lowpass_rate 0 = no lowpass, else its design rate. */
 dsdhip_cond_t *dsdhip_cond_create(int nch, int boxcar_taps, unsigned lowpass_rate);
+/* the same with the caller's lowpass taps (1..128) in place of the design */
+dsdhip_cond_t *dsdhip_cond_create_taps(int nch, int boxcar_taps, const double *h, int ntaps);
 void dsdhip_cond_destroy(dsdhip_cond_t *c);
 /* replaces the boxcar / lowpass part of engine_channel_reset (:732-757) */
 int dsdhip_cond_reset(dsdhip_cond_t *c);
