@@ -374,7 +374,7 @@ int dsdhip_trellis_process(dsdhip_trellis_t *t, int i, const double *in, float *
             if (hipMemcpy(bitsv.data(), d_out, sizeof(uint64_t) * nw, hipMemcpyDeviceToHost) != hipSuccess)
                 rc = -1;
             else {
-                for (int64_t k = 0; k < kept; k++)
+                for (int64_t k = 0; out && k < kept; k++)
                     out[k] = ((bitsv[(size_t)(k >> 6)] >> (k & 63)) & 1) ? 1.0f : -1.0f;
                 if (out_count) *out_count = (size_t)kept;
             }
@@ -383,6 +383,41 @@ int dsdhip_trellis_process(dsdhip_trellis_t *t, int i, const double *in, float *
     hipFree(d_in);
     hipFree(d_out);
     return rc;
+}
+
+/* sdm_drain (src/trellis.c:1042-1062): up to max_out outputs from zero
+ * input.  A context that has not filled its latency and is not draining
+ * first runs the missing fill steps on zeros without output; then `pending`
+ * zero steps (at most max_out) each give one output, and the context is
+ * left draining with pending reduced by that count. */
+int dsdhip_trellis_drain(dsdhip_trellis_t *t, int i, float *out, size_t max_out, size_t *out_count)
+{
+    if (!t || i < 0 || i >= t->nctx || (!out && max_out)) return -1;
+    tr_ctx c;
+    if (dsdhip_trellis_get(t, i, &c) != 0) return -1;
+    const int L = t->prm.L;
+    const size_t p0 = (size_t)c.pending;
+    const size_t len = p0 < max_out ? p0 : max_out;
+    if (!c.draining && c.pending < L) {
+        std::vector<double> z((size_t)(L - c.pending), 0.0);
+        size_t n = 0;
+        /* the fill steps of process produce no output */
+        if (dsdhip_trellis_process(t, i, z.data(), NULL, z.size(), &n) != 0 || n != 0) return -1;
+        if (dsdhip_trellis_get(t, i, &c) != 0) return -1;
+    }
+    size_t n = 0;
+    if (len) {
+        c.pending = L;                 /* every zero step gives an output */
+        if (dsdhip_trellis_put(t, i, &c) != 0) return -1;
+        std::vector<double> z(len, 0.0);
+        if (dsdhip_trellis_process(t, i, z.data(), out, len, &n) != 0 || n != len) return -1;
+        if (dsdhip_trellis_get(t, i, &c) != 0) return -1;
+    }
+    c.pending = (int32_t)(p0 - len);
+    c.draining = 1;
+    if (dsdhip_trellis_put(t, i, &c) != 0) return -1;
+    if (out_count) *out_count = n;
+    return 0;
 }
 
 } /* extern "C" */

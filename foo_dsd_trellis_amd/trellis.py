@@ -94,6 +94,13 @@ class SdmPool:
                                            x.size, C.byref(cnt)), "process")
         return out[:cnt.value]
 
+    def drain(self, i, max_out=1 << 20):
+        """sdm_drain: up to max_out float32 +-1.0 outputs from zero input."""
+        out = np.zeros(max(1, max_out), dtype=np.float32)
+        cnt = C.c_size_t(0)
+        check(lib().dsdhip_trellis_drain(self.h, i, out.ctypes.data_as(_fp), max_out, C.byref(cnt)), "drain")
+        return out[:cnt.value]
+
     def run_jobs(self, ctx, d_in, n_in, d_out, keep_from=None, copy_src=None, seed=None):
         """Batched launch on device buffers.
 

@@ -156,6 +156,9 @@ int dsdhip_trellis_best_state(dsdhip_trellis_t *t, int ctx, double *state10);
 /* replaces sdm_process_block (include/trellis.h:70-71), host buffers */
 int dsdhip_trellis_process(dsdhip_trellis_t *t, int ctx, const double *in, float *out,
                            size_t count, size_t *out_count);
+/* replaces sdm_drain (include/trellis.h:73): up to max_out outputs from zero
+ * input, flushing an unfilled latency first; host buffers */
+int dsdhip_trellis_drain(dsdhip_trellis_t *t, int ctx, float *out, size_t max_out, size_t *out_count);
 /* Batched form, device buffers (see dsdhip_trellis.cpp for field meaning):
  * one launch, one wave per job; outputs are packed bits (1 = +1.0). */
 int dsdhip_trellis_run_jobs(dsdhip_trellis_t *t, int njobs, const int *ctx, const int *copy_src,
