@@ -47,6 +47,8 @@ _SIGS = {
     "dsdhip_das_fma_o8": ([_vp, C.c_int], C.c_int),
     "dsdhip_trellis_wave_only": ([_vp, C.c_int], C.c_int),
     "dsdhip_trellis_drain": ([_vp, C.c_int, _fp, C.c_size_t, C.POINTER(C.c_size_t)], C.c_int),
+    "dsdhip_trellis_state_distance": ([_vp, C.c_int, C.c_int, _dp], C.c_int),
+    "dsdhip_estimate_state": ([_dp, _dp, C.c_int, _dp, _dp, C.c_size_t, C.c_double, C.c_int, _dp], C.c_int),
     "dsdhip_das_wave_only": ([_vp, C.c_int], C.c_int),
     "gpu_hip_fir_lowpass_setup": ([_vp, _fp, C.c_int], C.c_int),
     "gpu_hip_fir_lowpass_f64": ([_vp, _fp, _dp, C.c_size_t, C.c_double], C.c_int),

@@ -214,6 +214,47 @@ int dsdhip_das_force_literal(dsdhip_das_t *d, int on) { return d ? dsdhip_trelli
 int dsdhip_das_fma_o8(dsdhip_das_t *d, int on) { return d ? dsdhip_trellis_fma_o8(d->tr, on) : -1; }
 int dsdhip_das_wave_only(dsdhip_das_t *d, int on) { return d ? dsdhip_trellis_wave_only(d->tr, on) : -1; }
 
+/* sdm_estimate_state_multibit (src/trellis.c:1118-1187): the greedy
+ * num_levels quantizer run through the loop filter from init_state over
+ * count host samples, on one device lane (the DAS seeding kernel). */
+int dsdhip_estimate_state(const double *a, const double *g, int order, const double *init_state,
+                          const double *in, size_t count, double state_limit, int num_levels,
+                          double *out_state)
+{
+    if (!a || !g || !init_state || !out_state || (!in && count) || order < 2 || order > TR_MAXP ||
+        num_levels < 2)
+        return -1;
+    if (dsdhip_ensure_device() != 0) return -1;
+    double init[10] = {0}, ab[2 * TR_MAXP] = {0}, est[10];
+    memcpy(init, init_state, sizeof(double) * (size_t)order);
+    memcpy(ab, a, sizeof(double) * (size_t)order);
+    memcpy(ab + TR_MAXP, g, sizeof(double) * (size_t)order);
+    const int64_t bnd = (int64_t)count;
+    double *dx = NULL, *dinit = NULL, *dab = NULL, *dest = NULL;
+    int64_t *db = NULL;
+    int rc = -1;
+    if (hipMalloc(&dx, sizeof(double) * (count ? count : 1)) == hipSuccess &&
+        hipMalloc(&dinit, sizeof(init)) == hipSuccess && hipMalloc(&dab, sizeof(ab)) == hipSuccess &&
+        hipMalloc(&dest, sizeof(est)) == hipSuccess && hipMalloc(&db, sizeof(bnd)) == hipSuccess &&
+        (count == 0 || hipMemcpy(dx, in, sizeof(double) * count, hipMemcpyHostToDevice) == hipSuccess) &&
+        hipMemcpy(dinit, init, sizeof(init), hipMemcpyHostToDevice) == hipSuccess &&
+        hipMemcpy(dab, ab, sizeof(ab), hipMemcpyHostToDevice) == hipSuccess &&
+        hipMemcpy(db, &bnd, sizeof(bnd), hipMemcpyHostToDevice) == hipSuccess) {
+        hipLaunchKernelGGL(das_estimate, dim3(1), dim3(64), 0, 0, dx, (int64_t)count, (int64_t)0, (int64_t)count,
+                           db, 1, dinit, dab, dab + TR_MAXP, order, state_limit, num_levels, 1, dest);
+        if (hipGetLastError() == hipSuccess && hipMemcpy(est, dest, sizeof(est), hipMemcpyDeviceToHost) == hipSuccess) {
+            memcpy(out_state, est, sizeof(double) * (size_t)order);
+            rc = 0;
+        }
+    }
+    hipFree(dx);
+    hipFree(dinit);
+    hipFree(dab);
+    hipFree(dest);
+    hipFree(db);
+    return rc;
+}
+
 /* Plan for F samples: returns the segment count; fills start/size if given. */
 int dsdhip_das_plan(const dsdhip_das_t *d, size_t F, size_t *start, size_t *size)
 {

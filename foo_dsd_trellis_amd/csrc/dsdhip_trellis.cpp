@@ -385,6 +385,27 @@ int dsdhip_trellis_process(dsdhip_trellis_t *t, int i, const double *in, float *
     return rc;
 }
 
+/* sdm_state_distance (src/trellis.c:1094-1109): squared distance of the
+ * best survivors' loop-filter states of two contexts (1e30 if either has
+ * none). */
+int dsdhip_trellis_state_distance(dsdhip_trellis_t *t, int i, int j, double *dist)
+{
+    if (!t || !dist || i < 0 || j < 0 || i >= t->nctx || j >= t->nctx) return -1;
+    tr_ctx a, b;
+    if (dsdhip_trellis_get(t, i, &a) != 0 || dsdhip_trellis_get(t, j, &b) != 0) return -1;
+    if (a.num_cands < 1 || b.num_cands < 1) {
+        *dist = 1e30;
+        return 0;
+    }
+    double d = 0.0;
+    for (int k = 0; k < t->prm.P; k++) {
+        const double e = a.st[0][k] - b.st[0][k];
+        d += e * e;
+    }
+    *dist = d;
+    return 0;
+}
+
 /* sdm_drain (src/trellis.c:1042-1062): up to max_out outputs from zero
  * input.  A context that has not filled its latency and is not draining
  * first runs the missing fill steps on zeros without output; then `pending`

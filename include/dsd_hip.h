@@ -159,6 +159,14 @@ int dsdhip_trellis_process(dsdhip_trellis_t *t, int ctx, const double *in, float
 /* replaces sdm_drain (include/trellis.h:73): up to max_out outputs from zero
  * input, flushing an unfilled latency first; host buffers */
 int dsdhip_trellis_drain(dsdhip_trellis_t *t, int ctx, float *out, size_t max_out, size_t *out_count);
+/* replaces sdm_state_distance (include/trellis.h:82) for contexts i, j */
+int dsdhip_trellis_state_distance(dsdhip_trellis_t *t, int i, int j, double *dist);
+/* replaces sdm_estimate_state_multibit (include/trellis.h:114-118); with
+ * num_levels = 16 it is sdm_estimate_state (:108-110).  Host buffers; the
+ * greedy recurrence runs on one device lane. */
+int dsdhip_estimate_state(const double *a, const double *g, int order, const double *init_state,
+                          const double *in, size_t count, double state_limit, int num_levels,
+                          double *out_state);
 /* Batched form, device buffers (see dsdhip_trellis.cpp for field meaning):
  * one launch, one wave per job; outputs are packed bits (1 = +1.0). */
 int dsdhip_trellis_run_jobs(dsdhip_trellis_t *t, int njobs, const int *ctx, const int *copy_src,
