@@ -95,6 +95,24 @@ __device__ __forceinline__ unsigned shl_add_lt(unsigned acc, double k, double c)
     return r;
 }
 
+/* four steps of shl_add_ltu (keys k.w, k.z, k.y, k.x in that order) in one
+ * asm statement: the compiler pads every asm statement with a wait state */
+__device__ __forceinline__ unsigned shl_add_ltu4(unsigned acc, uint4 k, unsigned c)
+{
+    __asm__("v_cmp_lt_u32_e32 vcc, %1, %5\n\t"
+            "v_addc_co_u32_e32 %0, vcc, %0, %0, vcc\n\t"
+            "v_cmp_lt_u32_e32 vcc, %2, %5\n\t"
+            "v_addc_co_u32_e32 %0, vcc, %0, %0, vcc\n\t"
+            "v_cmp_lt_u32_e32 vcc, %3, %5\n\t"
+            "v_addc_co_u32_e32 %0, vcc, %0, %0, vcc\n\t"
+            "v_cmp_lt_u32_e32 vcc, %4, %5\n\t"
+            "v_addc_co_u32_e32 %0, vcc, %0, %0, vcc"
+            : "+v"(acc)
+            : "v"(k.w), "v"(k.z), "v"(k.y), "v"(k.x), "v"(c)
+            : "vcc");
+    return acc;
+}
+
 /* acc * 2 + (k <= c) */
 __device__ __forceinline__ unsigned shl_add_le(unsigned acc, double k, double c)
 {
@@ -135,12 +153,7 @@ __device__ __forceinline__ u64 lt_sweep_hi(const uint32_t *key, unsigned c)
         for (int i = 3; i >= 0; i--) kv[i] = k4[b * 4 + i];
         unsigned acc = w[b >> 1];
 #pragma unroll
-        for (int i = 3; i >= 0; i--) {
-            acc = shl_add_ltu(acc, kv[i].w, c);
-            acc = shl_add_ltu(acc, kv[i].z, c);
-            acc = shl_add_ltu(acc, kv[i].y, c);
-            acc = shl_add_ltu(acc, kv[i].x, c);
-        }
+        for (int i = 3; i >= 0; i--) acc = shl_add_ltu4(acc, kv[i], c);
         w[b >> 1] = acc;
         __builtin_amdgcn_sched_barrier(0);
     }
