@@ -57,10 +57,19 @@ def traffic_per_input_sample():
 
 
 def cpu_baseline(x_seg, fi, threads, target_s):
-    """Oracle trellis segments (K=32, L=64) on `threads` host threads."""
+    """Trellis segments (K=32, L=64) on `threads` host threads: the
+    reference's own src/trellis.c compiled with gcc -O2 -ffp-contract=off
+    (oracle/_ref) where it was built, else the oracle's C restatement."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
-    O = oracle_lib.oracle()
+    O = oracle_lib.reference()
+    kind, what = "reference", "reference src/trellis.c (gcc -O2, scalar)"
+    if O is None:
+        O, kind, what = oracle_lib.oracle(), "port", "oracle C restatement"
+    name = oracle_lib.oracle().ntf_get(fi)["name"]
+    fi = O.ntf_auto(DSD256)
+    if O.ntf_get(fi)["name"] != name:
+        raise RuntimeError("CPU baseline NTF %s differs from %s" % (O.ntf_get(fi)["name"], name))
     keep = SEG_LEN
     # calibrate one segment on one thread
     t0 = time.perf_counter()
@@ -85,10 +94,10 @@ def cpu_baseline(x_seg, fi, threads, target_s):
     wall = time.perf_counter() - t0
     total = sum(done)
     return {"value": round(total / wall / 1e6, 4), "unit": "Msamples/s", "cores": threads,
-            "kind": "port",
+            "kind": kind,
             "sample": "%d threads x %d trellis segments of %d inputs (%d kept outputs each), "
-                      "CLANS-8 K=32 L=64 B=8, oracle C restatement, %.1f s wall"
-                      % (threads, per_thread, x_seg.size, keep, wall)}
+                      "CLANS-8 K=32 L=64 B=8, %s, %.1f s wall"
+                      % (threads, per_thread, x_seg.size, keep, what, wall)}
 
 
 def diff_loop_filter(steps, check_rows):
@@ -409,15 +418,20 @@ def main():
         n_out = step(timed=True)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    # FIR chain (8 half-band stages): algorithmic bytes = first-stage float
-    # input + every stage's double output + the double inputs of stages 2..8
+    # FIR chain (8 half-band stages, one launch each): bytes = first-stage
+    # float input + every stage's double output + the double inputs of
+    # stages 2..8 (what a stage-per-launch chain must move); compulsory
+    # bytes = the float input + the final double output only
     fir_ms = sum(a.elapsed_time(b) for a, b in fir_ev) / max(1, len(fir_ev))
     nst = int(round(np.log2(DSD256 // FS_IN)))
     fir_bytes = NCH * n_in * (4 + 16 * (2 ** nst - 1) + 8 * (2 ** nst - 2))
+    fir_min = NCH * n_in * (4 + 8 * 2 ** nst)
     fir_roof = {"bound": "hbm", "ms": round(fir_ms, 3), "bytes": int(fir_bytes),
                 "achieved_GBps": round(fir_bytes / (fir_ms / 1e3) / 1e9, 1) if fir_ms > 0 else None,
                 "peak_GBps": 8000.0,
                 "frac": round(fir_bytes / (fir_ms / 1e3) / 8e12, 4) if fir_ms > 0 else None,
+                "compulsory_bytes": int(fir_min),
+                "frac_compulsory": round(fir_min / (fir_ms / 1e3) / 8e12, 4) if fir_ms > 0 else None,
                 "stages": nst, "note": "whole chain per step on the default stream, HIP events"}
     k_ms, k_launches, k_samples = D.timing_read(clear=True)
     ms_step = (t1 - t0) * 1000.0 / args.steps

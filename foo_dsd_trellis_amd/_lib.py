@@ -105,12 +105,30 @@ _SIGS = {
 _lib = None
 
 
+def _torch_runtime_first():
+    """PyTorch for ROCm carries its own HIP runtime (torch/lib/libamdhip64.so)
+    beside the system runtime this library links, so a process that uses
+    both holds two HIP runtimes.  Torch's is brought up first: started
+    after the library's runtime had already run work on the device, torch's
+    lazy initialisation has reported no GPU."""
+    try:
+        import torch
+    except ImportError:
+        return
+    try:
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except Exception:
+        pass
+
+
 def lib():
     """The loaded library (raises DsdHipError if it is not built)."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise DsdHipError("libdsdhip.so is not built (run __graft_entry__.build())")
+        _torch_runtime_first()
         L = C.CDLL(LIB_PATH)
         for name, (args, res) in _SIGS.items():
             fn = getattr(L, name, None)

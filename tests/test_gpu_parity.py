@@ -37,6 +37,7 @@ TRELLIS_CASES = [
     (4, 2048, 8, 2822400, 7000, "pink"),      # longest history (32 words per survivor)
     (2, 129, 4, 11289600, 9000, "pink"),      # K = 2 past the lane kernel's history limit
     (32, 64, 32, 11289600, 6000, "loud"),     # 32-bit paths
+    (32, 2048, 8, 11289600, 5000, "loud"),    # largest K and L together
 ]
 
 
@@ -177,3 +178,52 @@ def test_equal_metrics_in_parallel_prune(gpu, oracle, K):
     assert np.array_equal(y, ref)
     if K < 32:
         assert slow[0] == 0
+
+
+def test_trellis_empty_and_tiny_calls(gpu, oracle):
+    """Empty and one-sample calls between blocks change nothing: the output
+    stream equals the oracle's over the concatenated input (the reference's
+    sdm_process_block takes any count, src/trellis.c:1013-1040)."""
+    fi = oracle.ntf_auto(11289600)
+    x = _x("pink", 9000, 11289600, 13)
+    ref = oracle.ctx(fi, 8, 32, 64).process(x)
+    p = T.SdmPool(fi, 8, 32, 64, 0.0, 1)
+    ys = []
+    for a, b in [(0, 0), (0, 1), (1, 1), (1, 64), (64, 65), (65, 65), (65, 9000), (9000, 9000)]:
+        ys.append(p.process(0, x[a:b]))
+    assert [y.size for y in ys[:3]] == [0, 0, 0]
+    assert np.array_equal(np.concatenate(ys), ref)
+
+
+def test_seam_trellis_short_calls(gpu, oracle):
+    """gpu_hip_trellis, like gpu_cuda_trellis (src/gpu_cuda.c:1434-1437),
+    has no minimum count (the plugin checks GPU_MIN_SAMPLES before calling,
+    src/dsp_plugin.c:1405): short and empty calls continue the stream
+    exactly.  gpu_hip_trellis_das with no samples returns -1."""
+    import ctypes as C
+    from foo_dsd_trellis_amd._lib import lib
+    L = lib()
+    h = L.gpu_hip_create()
+    assert h
+    try:
+        fi = oracle.ntf_auto(11289600)
+        f = oracle.ntf_get(fi)
+        a = np.ascontiguousarray(f["a"][:10])
+        g = np.ascontiguousarray(f["g"][:10])
+        dp = lambda v: v.ctypes.data_as(C.POINTER(C.c_double))
+        fp = lambda v: v.ctypes.data_as(C.POINTER(C.c_float))
+        assert L.gpu_hip_trellis_setup(h, 8, f["order"], 16, dp(a), dp(g), 0.0, 8) == 0
+        x = _x("pink", 3000, 11289600, 17)
+        ref = oracle.ctx(fi, 8, 8, 16).process(x)
+        got = []
+        for a0, b0 in [(0, 0), (0, 5), (5, 16), (16, 17), (17, 17), (17, 3000)]:
+            part = np.ascontiguousarray(x[a0:b0])
+            out = np.zeros(max(1, b0 - a0), np.float32)
+            assert L.gpu_hip_trellis(h, dp(part if part.size else np.zeros(1)), fp(out), b0 - a0) == 0
+            got.append(out[:max(0, b0 - max(a0, 16))])
+        assert np.array_equal(np.concatenate(got), ref)
+        z = np.zeros(1)
+        o = np.zeros(1, np.float32)
+        assert L.gpu_hip_trellis_das(h, dp(z), fp(o), 0, 1) == -1
+    finally:
+        L.gpu_hip_destroy(h)
