@@ -9,6 +9,7 @@
  *   DAS candidate seeding         (src/dsp_plugin.c:2329-2338)         -> dsdhip_trellis_seed
  */
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <stdlib.h>
 #include <string.h>
 #include <vector>
@@ -76,7 +77,21 @@ dsdhip_trellis_t *dsdhip_trellis_create_coeffs(const double *a, const double *g,
         delete t;
         return NULL;
     }
-    for (int i = 0; i < nctx; i++) dsdhip_trellis_reset(t, i);
+    /* every context starts fresh: one host copy, then doubling
+     * device-to-device copies (a DAS pool holds thousands of contexts) */
+    tr_ctx c;
+    ctx_fresh(&c);
+    bool ok = hipMemcpy(&t->d_ctx[0], &c, sizeof(c), hipMemcpyHostToDevice) == hipSuccess;
+    for (int done = 1; ok && done < nctx; done *= 2) {
+        const int n = std::min(done, nctx - done);
+        ok = hipMemcpy(&t->d_ctx[done], &t->d_ctx[0], sizeof(tr_ctx) * (size_t)n,
+                       hipMemcpyDeviceToDevice) == hipSuccess;
+    }
+    if (!ok) {
+        hipFree(t->d_ctx);
+        delete t;
+        return NULL;
+    }
     return t;
 }
 
