@@ -21,6 +21,11 @@ def _x(kind, n, rate, seed):
         return np.clip(0.18 * SG.pink(n, rng), -0.5, 0.5)
     if kind == "loud":   # drives bucket-0 lookups after the act[32] write at K=32
         return 0.45 * np.sin(2 * np.pi * 997 * 16 * t) + 0.05 * rng.standard_normal(n)
+    if kind == "quiet":  # pink with near-silent stretches over segment boundaries
+        x = np.clip(0.18 * SG.pink(n, rng), -0.5, 0.5)
+        for a in (30000, 62000):
+            x[a:a + 6000] *= 1e-3
+        return x
     raise ValueError(kind)
 
 
@@ -126,6 +131,8 @@ DAS_CASES = [
     (32, 64, 8, 11289600, 120000, 16384, 0, 0, "pink"),
     (32, 64, 8, 11289600, 120000, 16384, 0, 1, "pink"),
     (16, 32, 8, 5644800, 90000, 0, 5, 1, "loud"),
+    (32, 64, 8, 11289600, 120000, 16384, 0, 0, "quiet"),   # boundaries moved out of silence
+    (8, 16, 8, 2822400, 120000, 16384, 0, 1, "quiet"),
 ]
 
 
@@ -140,6 +147,8 @@ def test_das_bit_exact(gpu, oracle, K, L, B, rate, F, seg_len, nseg, seg0, kind)
     y, m = d.run(x)
     S, slow, pos = d.last_info(len(sp))
     assert S == len(sp)
+    if kind == "quiet":   # the quiet-boundary reposition (src/dsp_plugin.c:1963-2034) took place
+        assert any(plan[i, 0] != i * seg_len for i in range(2, S)), plan[:, 0].tolist()
     assert m == n
     assert list(pos[1:]) == list(sp[1:]) or S == 1
     for c in range(nch):
