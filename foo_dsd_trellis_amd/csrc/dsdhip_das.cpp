@@ -602,16 +602,22 @@ static int run_tracks(dsdhip_das_t *d, const double *const *d_in, const int64_t 
             double ab[2 * TR_MAXP];
             memcpy(ab, prm.a, sizeof(prm.a));
             memcpy(ab + TR_MAXP, prm.g, sizeof(prm.g));
-            hipMemcpyAsync(db.p, bnd.data(), sizeof(int64_t) * bnd.size(), hipMemcpyHostToDevice, d->stream);
-            hipMemcpyAsync(dinit.p, init.data(), sizeof(double) * init.size(), hipMemcpyHostToDevice, d->stream);
-            hipMemcpyAsync(dab.p, ab, sizeof(ab), hipMemcpyHostToDevice, d->stream);
+            bool ok =
+                hipMemcpyAsync(db.p, bnd.data(), sizeof(int64_t) * bnd.size(), hipMemcpyHostToDevice,
+                               d->stream) == hipSuccess &&
+                hipMemcpyAsync(dinit.p, init.data(), sizeof(double) * init.size(), hipMemcpyHostToDevice,
+                               d->stream) == hipSuccess &&
+                hipMemcpyAsync(dab.p, ab, sizeof(ab), hipMemcpyHostToDevice, d->stream) == hipSuccess;
+            if (!ok) return -1;
             hipLaunchKernelGGL(das_estimate, dim3((unsigned)((nch + 63) / 64)), dim3(64), 0, d->stream, r.in,
                                r.in_stride, es, (int64_t)r.F - es, db.p, (int)bnd.size(), dinit.p,
                                dab.p, dab.p + TR_MAXP, prm.P, prm.state_limit, 16, nch, dest.p);
+            if (hipGetLastError() != hipSuccess) return -1;
             est[(size_t)t].resize((size_t)nch * bnd.size() * 10);
-            hipMemcpyAsync(est[(size_t)t].data(), dest.p, sizeof(double) * est[(size_t)t].size(),
-                           hipMemcpyDeviceToHost, d->stream);
-            if (hipStreamSynchronize(d->stream) != hipSuccess) return -1;
+            if (hipMemcpyAsync(est[(size_t)t].data(), dest.p, sizeof(double) * est[(size_t)t].size(),
+                               hipMemcpyDeviceToHost, d->stream) != hipSuccess ||
+                hipStreamSynchronize(d->stream) != hipSuccess)
+                return -1;
         }
         for (int t = 0; t < T; t++) {
             TrackRun &r = R[(size_t)t];
@@ -678,20 +684,31 @@ static int run_tracks(dsdhip_das_t *d, const double *const *d_in, const int64_t 
             dnc.ensure((size_t)nbd) || dpk.ensure((size_t)nbd) || dpos.ensure((size_t)nbd * QN) ||
             dden.ensure((size_t)nbd * QN) || dmx.ensure((size_t)nbd * QN))
             return -1;
-        hipMemcpyAsync(dpb.p, pbit.data(), sizeof(int64_t) * (size_t)nbd, hipMemcpyHostToDevice, d->stream);
-        hipMemcpyAsync(dcb.p, cbit.data(), sizeof(int64_t) * (size_t)nbd, hipMemcpyHostToDevice, d->stream);
-        hipMemcpyAsync(dov.p, ovl.data(), sizeof(int) * (size_t)nbd, hipMemcpyHostToDevice, d->stream);
+        bool ok =
+            hipMemcpyAsync(dpb.p, pbit.data(), sizeof(int64_t) * (size_t)nbd, hipMemcpyHostToDevice,
+                           d->stream) == hipSuccess &&
+            hipMemcpyAsync(dcb.p, cbit.data(), sizeof(int64_t) * (size_t)nbd, hipMemcpyHostToDevice,
+                           d->stream) == hipSuccess &&
+            hipMemcpyAsync(dov.p, ovl.data(), sizeof(int) * (size_t)nbd, hipMemcpyHostToDevice,
+                           d->stream) == hipSuccess;
+        if (!ok) return -1;
         hipLaunchKernelGGL(das_stitch, dim3((unsigned)nbd), dim3(256), 0, d->stream,
                            segbits.p, dpb.p, dcb.p, dov.p, L, d->d_qfir, dnc.p, dpk.p, dpos.p,
                            dden.p, dmx.p);
+        if (hipGetLastError() != hipSuccess) return -1;
         std::vector<int> nc((size_t)nbd), pk((size_t)nbd), pos((size_t)nbd * QN), den((size_t)nbd * QN);
         std::vector<double> mx((size_t)nbd * QN);
-        hipMemcpyAsync(nc.data(), dnc.p, sizeof(int) * nc.size(), hipMemcpyDeviceToHost, d->stream);
-        hipMemcpyAsync(pk.data(), dpk.p, sizeof(int) * pk.size(), hipMemcpyDeviceToHost, d->stream);
-        hipMemcpyAsync(pos.data(), dpos.p, sizeof(int) * pos.size(), hipMemcpyDeviceToHost, d->stream);
-        hipMemcpyAsync(den.data(), dden.p, sizeof(int) * den.size(), hipMemcpyDeviceToHost, d->stream);
-        hipMemcpyAsync(mx.data(), dmx.p, sizeof(double) * mx.size(), hipMemcpyDeviceToHost, d->stream);
-        if (hipStreamSynchronize(d->stream) != hipSuccess) return -1;
+        ok = hipMemcpyAsync(nc.data(), dnc.p, sizeof(int) * nc.size(), hipMemcpyDeviceToHost,
+                            d->stream) == hipSuccess &&
+             hipMemcpyAsync(pk.data(), dpk.p, sizeof(int) * pk.size(), hipMemcpyDeviceToHost,
+                            d->stream) == hipSuccess &&
+             hipMemcpyAsync(pos.data(), dpos.p, sizeof(int) * pos.size(), hipMemcpyDeviceToHost,
+                            d->stream) == hipSuccess &&
+             hipMemcpyAsync(den.data(), dden.p, sizeof(int) * den.size(), hipMemcpyDeviceToHost,
+                            d->stream) == hipSuccess &&
+             hipMemcpyAsync(mx.data(), dmx.p, sizeof(double) * mx.size(), hipMemcpyDeviceToHost,
+                            d->stream) == hipSuccess;
+        if (hipStreamSynchronize(d->stream) != hipSuccess || !ok) return -1;
         for (int i = 0; i < nbd; i++) {
             const int n = nc[(size_t)i];
             const int peak = pk[(size_t)i];
