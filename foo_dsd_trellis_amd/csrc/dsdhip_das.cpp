@@ -43,6 +43,42 @@ extern "C" __global__ void bits_to_f32(const unsigned long long *, int64_t, int6
 
 static double nan_val() { return __builtin_nan(""); }
 
+/* device buffer that only grows; freed with its owner */
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n)
+    {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    int ensure(size_t m)
+    {
+        if (m <= n) return 0;
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+        if (hipMalloc(&p, sizeof(T) * (m ? m : 1)) != hipSuccess) return -1;
+        n = m;
+        return 0;
+    }
+    ~DevBuf() { if (p) hipFree(p); }
+};
+
+/* per-object device scratch of a run, kept between calls (grow-only); like
+ * the trellis contexts, one object is used from one thread at a time */
+struct DasScratch {
+    DevBuf<int64_t> dc, db, dpb, dcb;
+    DevBuf<double> ds, dinit, dest, dab, dmx;
+    DevBuf<int> dov, dnc, dpk, dpos, dden;
+    DevBuf<unsigned long long> segbits;
+    std::vector<DevBuf<int64_t>> dsb, ddb, dlb;   /* per track */
+};
+
 struct dsdhip_das {
     double a[10], g[10];
     int order, depth, K, L, nch;
@@ -61,6 +97,7 @@ struct dsdhip_das {
     int last_S;
     std::vector<int> last_stitch;
     int64_t last_slow;
+    DasScratch sc;
 };
 
 /* 255-tap Kaiser (beta 8) low-pass used to score stitch candidates
@@ -99,22 +136,6 @@ static inline int ix_persist(const dsdhip_das_t *, int s) { return s; }
 static inline int ix_snap(const dsdhip_das_t *d, int s) { return d->ns + s; }
 static inline int ix_seg(const dsdhip_das_t *d, int s, int seg) { return 2 * d->ns + s * d->cap_seg + seg; }
 
-template <typename T>
-struct DevBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    int ensure(size_t m)
-    {
-        if (m <= n) return 0;
-        if (p) hipFree(p);
-        p = nullptr;
-        n = 0;
-        if (hipMalloc(&p, sizeof(T) * (m ? m : 1)) != hipSuccess) return -1;
-        n = m;
-        return 0;
-    }
-    ~DevBuf() { if (p) hipFree(p); }
-};
 
 extern "C" {
 
@@ -366,8 +387,8 @@ static int reposition(dsdhip_das_t *d, const double *d_ch0, size_t F, std::vecto
         segs.push_back(seg);
     }
     if (centers.empty()) return 0;
-    DevBuf<int64_t> dc;
-    DevBuf<double> ds;
+    DevBuf<int64_t> &dc = d->sc.dc;
+    DevBuf<double> &ds = d->sc.ds;
     std::vector<double> sums(centers.size());
     auto run = [&](const std::vector<int64_t> &cv, std::vector<double> &out) -> int {
         out.resize(cv.size());
@@ -531,7 +552,7 @@ static int run_tracks(dsdhip_das_t *d, const double *const *d_in, const int64_t 
         r.kept.assign((size_t)(nch * S), 0);
         r.slow.assign((size_t)(nch * S), 0);
     }
-    DevBuf<unsigned long long> segbits;
+    DevBuf<unsigned long long> &segbits = d->sc.segbits;
     if (segbits.ensure((size_t)wtot)) return -1;
 
     struct JobRef { int t, ch, seg, src; };
@@ -594,8 +615,8 @@ static int run_tracks(dsdhip_das_t *d, const double *const *d_in, const int64_t 
             const int64_t es = (int64_t)r.st[1];
             std::vector<int64_t> bnd;
             for (int seg = 1; seg < S; seg++) bnd.push_back(seg >= 2 ? (int64_t)r.st[(size_t)seg] - es : 0);
-            DevBuf<int64_t> db;
-            DevBuf<double> dinit, dest, dab;
+            DevBuf<int64_t> &db = d->sc.db;
+            DevBuf<double> &dinit = d->sc.dinit, &dest = d->sc.dest, &dab = d->sc.dab;
             if (db.ensure(bnd.size()) || dinit.ensure(init.size()) ||
                 dest.ensure((size_t)nch * bnd.size() * 10) || dab.ensure(2 * TR_MAXP))
                 return -1;
@@ -677,9 +698,10 @@ static int run_tracks(dsdhip_das_t *d, const double *const *d_in, const int64_t 
     }
     const int nbd = (int)pbit.size();
     if (nbd > 0) {
-        DevBuf<int64_t> dpb, dcb;
-        DevBuf<int> dov, dnc, dpk, dpos, dden;
-        DevBuf<double> dmx;
+        DevBuf<int64_t> &dpb = d->sc.dpb, &dcb = d->sc.dcb;
+        DevBuf<int> &dov = d->sc.dov, &dnc = d->sc.dnc, &dpk = d->sc.dpk, &dpos = d->sc.dpos,
+                    &dden = d->sc.dden;
+        DevBuf<double> &dmx = d->sc.dmx;
         if (dpb.ensure((size_t)nbd) || dcb.ensure((size_t)nbd) || dov.ensure((size_t)nbd) ||
             dnc.ensure((size_t)nbd) || dpk.ensure((size_t)nbd) || dpos.ensure((size_t)nbd * QN) ||
             dden.ensure((size_t)nbd * QN) || dmx.ensure((size_t)nbd * QN))
@@ -730,7 +752,12 @@ static int run_tracks(dsdhip_das_t *d, const double *const *d_in, const int64_t 
     }
 
     /* ---- assemble each track (same positions on every channel) ---- */
-    std::vector<DevBuf<int64_t>> dsb((size_t)T), ddb((size_t)T), dlb((size_t)T);
+    if (d->sc.dsb.size() < (size_t)T) {
+        d->sc.dsb.resize((size_t)T);
+        d->sc.ddb.resize((size_t)T);
+        d->sc.dlb.resize((size_t)T);
+    }
+    std::vector<DevBuf<int64_t>> &dsb = d->sc.dsb, &ddb = d->sc.ddb, &dlb = d->sc.dlb;
     for (int t = 0; t < T; t++) {
         TrackRun &r = R[(size_t)t];
         const int S = r.S;
