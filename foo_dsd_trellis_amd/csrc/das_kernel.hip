@@ -371,16 +371,29 @@ extern "C" __global__ void das_assemble(const u64 *__restrict__ src, const int64
         const int mid = (lo + hi) >> 1;
         if (dst_bit[base + mid] <= b0) { s = mid; lo = mid + 1; } else hi = mid - 1;
     }
+    /* the word's bits come in runs, one per span piece: each run is one
+     * funnel shift of (at most) two source words; bits no span covers are 0 */
     u64 word = 0;
-    for (int i = 0; i < 64; i++) {
-        const int64_t ob = b0 + i;
-        if (ob >= total_bits) break;
+    const int64_t end = (b0 + 64 < total_bits) ? b0 + 64 : total_bits;
+    int64_t ob = b0;
+    while (ob < end) {
         while (s + 1 < nspan_per_ch && dst_bit[base + s + 1] <= ob) s++;
-        const int64_t d0 = dst_bit[base + s];
-        if (ob >= d0 && ob < d0 + len[base + s]) {
-            const int64_t sb = src_bit[base + s] + (ob - d0);
-            word |= (u64)((src[sb >> 6] >> (sb & 63)) & 1ull) << i;
+        const int64_t d0 = dst_bit[base + s], l = len[base + s];
+        if (ob < d0 || ob >= d0 + l) {      /* gap: up to the next span start */
+            int64_t nxt = (ob < d0) ? d0 : ((s + 1 < nspan_per_ch) ? dst_bit[base + s + 1] : end);
+            ob = (nxt > ob && nxt < end) ? nxt : end;
+            continue;
         }
+        const int64_t run_end = (d0 + l < end) ? d0 + l : end;
+        const int n = (int)(run_end - ob);  /* 1..64 */
+        const int64_t sb = src_bit[base + s] + (ob - d0);
+        const int64_t sw = sb >> 6;
+        const int off = (int)(sb & 63);
+        u64 v = src[sw] >> off;
+        if (off != 0 && n > 64 - off) v |= src[sw + 1] << (64 - off);
+        if (n < 64) v &= (1ull << n) - 1ull;
+        word |= v << (ob - b0);
+        ob = run_end;
     }
     out[(int64_t)ch * out_words_per_ch + w] = word;
 }
