@@ -77,6 +77,9 @@ struct DasScratch {
     DevBuf<int> dov, dnc, dpk, dpos, dden;
     DevBuf<unsigned long long> segbits;
     std::vector<DevBuf<int64_t>> dsb, ddb, dlb;   /* per track */
+    DevBuf<double> hin;                           /* host-buffer entry: staged input */
+    DevBuf<unsigned long long> hbits;             /* ... packed output */
+    DevBuf<float> hout;                           /* ... +-1.0 output */
 };
 
 struct dsdhip_das {
@@ -861,30 +864,23 @@ int dsdhip_das_last_info(const dsdhip_das_t *d, int *nseg, int64_t *slow_steps, 
 int dsdhip_das_run(dsdhip_das_t *d, const double *in, size_t F, float *out, size_t *out_count)
 {
     if (!d) return -1;
+    if (F == 0 || !in || !out) return -1;
     const int nch = d->nch;
     const int64_t words = (int64_t)(F + 63) / 64 + 1;
-    double *din = NULL;
-    unsigned long long *dbits = NULL;
-    float *dout = NULL;
-    int rc = -1;
+    DasScratch &sc = d->sc;
     size_t cnt = 0;
-    if (hipMalloc(&din, sizeof(double) * F * (size_t)nch) == hipSuccess &&
-        hipMalloc(&dbits, sizeof(unsigned long long) * (size_t)words * (size_t)nch) == hipSuccess &&
-        hipMalloc(&dout, sizeof(float) * F * (size_t)nch) == hipSuccess &&
-        hipMemcpy(din, in, sizeof(double) * F * (size_t)nch, hipMemcpyHostToDevice) == hipSuccess &&
-        dsdhip_das_run_dev(d, din, (int64_t)F, F, dbits, words, &cnt) == 0) {
-        hipLaunchKernelGGL(bits_to_f32, dim3((unsigned)((cnt + 255) / 256 + 1), (unsigned)nch), dim3(256),
-                           0, d->stream, dbits, words, (int64_t)cnt, dout, (int64_t)F);
-        if (hipStreamSynchronize(d->stream) == hipSuccess &&
-            hipMemcpy(out, dout, sizeof(float) * F * (size_t)nch, hipMemcpyDeviceToHost) == hipSuccess) {
-            rc = 0;
-            if (out_count) *out_count = cnt;
-        }
-    }
-    if (din) hipFree(din);
-    if (dbits) hipFree(dbits);
-    if (dout) hipFree(dout);
-    return rc;
+    if (sc.hin.ensure(F * (size_t)nch) || sc.hbits.ensure((size_t)words * (size_t)nch) ||
+        sc.hout.ensure(F * (size_t)nch) ||
+        hipMemcpy(sc.hin.p, in, sizeof(double) * F * (size_t)nch, hipMemcpyHostToDevice) != hipSuccess ||
+        dsdhip_das_run_dev(d, sc.hin.p, (int64_t)F, F, sc.hbits.p, words, &cnt) != 0)
+        return -1;
+    hipLaunchKernelGGL(bits_to_f32, dim3((unsigned)((cnt + 255) / 256 + 1), (unsigned)nch), dim3(256), 0,
+                       d->stream, sc.hbits.p, words, (int64_t)cnt, sc.hout.p, (int64_t)F);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(d->stream) != hipSuccess ||
+        hipMemcpy(out, sc.hout.p, sizeof(float) * F * (size_t)nch, hipMemcpyDeviceToHost) != hipSuccess)
+        return -1;
+    if (out_count) *out_count = cnt;
+    return 0;
 }
 
 } /* extern "C" */
