@@ -236,3 +236,21 @@ def test_seam_trellis_short_calls(gpu, oracle):
         assert L.gpu_hip_trellis_das(h, dp(z), fp(o), 0, 1) == -1
     finally:
         L.gpu_hip_destroy(h)
+
+
+@pytest.mark.parametrize("nch,F,nseg,seg0", [(2, 40000, 1, 1), (1, 40001, 1, 0), (2, 40000, 2, 1),
+                                             (1, 65599, 2, 1)])
+def test_das_reduces_to_sequential_reference(gpu, oracle, reference, nch, F, nseg, seg0):
+    """One segment, or two with seg0-first seeding (segment 1 continues the
+    persistent context exactly, src/dsp_plugin.c:2185-2250), is the plain
+    sequential trellis: the GPU DAS output equals the reference's own
+    trellis.c over each whole channel (ragged F and one channel included)."""
+    rate = 2822400
+    x = np.stack([_x("pink", F, rate, 31 + c) for c in range(nch)])
+    fi = oracle.ntf_auto(rate)
+    d = D.Das(fi, 8, 8, 16, nch, seg_len=0, nseg=nseg, seg0_first=bool(seg0), reposition=True)
+    y, m = d.run(x)
+    for c in range(nch):
+        seq = reference.ctx(fi, 8, 8, 16).process(x[c])
+        assert m == seq.size
+        assert np.array_equal(y[c, :m], seq), c
