@@ -352,7 +352,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=1.0,
-                    help="per-thread CPU oracle work for the cpu_baseline leg (0: skip)")
+                    help="seconds of CPU work per thread for the cpu_baseline leg: the reference trellis.c where built, else the oracle (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--plain-seconds", type=float, default=0.05,
                     help="audio seconds per channel for the plain per-channel GPU form (0: skip)")
