@@ -33,6 +33,9 @@ struct dsdhip_fir {
     double *d_buf[2];
     size_t buf_cap;   /* doubles per buffer */
     hipStream_t stream;
+    float *d_hin;     /* host-buffer entry: staged input / output, grow-only */
+    double *d_hout;
+    size_t hin_cap, hout_cap;
 };
 
 /* Modified Bessel I0 by its power series, 25 terms with early exit
@@ -144,6 +147,8 @@ void dsdhip_fir_destroy(dsdhip_fir_t *f)
         if (f->d_hist[s]) hipFree(f->d_hist[s]);
     }
     for (int b = 0; b < 2; b++) if (f->d_buf[b]) hipFree(f->d_buf[b]);
+    if (f->d_hin) hipFree(f->d_hin);
+    if (f->d_hout) hipFree(f->d_hout);
     free(f);
 }
 
@@ -237,19 +242,27 @@ int dsdhip_fir_process(dsdhip_fir_t *f, const float *in, size_t n, double *out)
 {
     if (!f) return -1;
     const size_t nout = n << f->nst;
-    float *d_in = NULL;
-    double *d_out = NULL;
-    int rc = -1;
-    if (hipMalloc(&d_in, sizeof(float) * n * (size_t)f->nch + 4) == hipSuccess &&
-        hipMalloc(&d_out, sizeof(double) * nout * (size_t)f->nch + 8) == hipSuccess &&
-        hipMemcpy(d_in, in, sizeof(float) * n * (size_t)f->nch, hipMemcpyHostToDevice) == hipSuccess &&
-        dsdhip_fir_process_dev(f, d_in, (int64_t)n, n, d_out, (int64_t)nout) == 0 &&
-        hipStreamSynchronize(f->stream) == hipSuccess &&
-        hipMemcpy(out, d_out, sizeof(double) * nout * (size_t)f->nch, hipMemcpyDeviceToHost) == hipSuccess)
-        rc = 0;
-    if (d_in) hipFree(d_in);
-    if (d_out) hipFree(d_out);
-    return rc;
+    const size_t nin_all = n * (size_t)f->nch + 1, nout_all = nout * (size_t)f->nch + 1;
+    if (f->hin_cap < nin_all) {
+        if (f->d_hin) hipFree(f->d_hin);
+        f->d_hin = NULL;
+        f->hin_cap = 0;
+        if (hipMalloc(&f->d_hin, sizeof(float) * nin_all) != hipSuccess) return -1;
+        f->hin_cap = nin_all;
+    }
+    if (f->hout_cap < nout_all) {
+        if (f->d_hout) hipFree(f->d_hout);
+        f->d_hout = NULL;
+        f->hout_cap = 0;
+        if (hipMalloc(&f->d_hout, sizeof(double) * nout_all) != hipSuccess) return -1;
+        f->hout_cap = nout_all;
+    }
+    if (hipMemcpy(f->d_hin, in, sizeof(float) * n * (size_t)f->nch, hipMemcpyHostToDevice) != hipSuccess ||
+        dsdhip_fir_process_dev(f, f->d_hin, (int64_t)n, n, f->d_hout, (int64_t)nout) != 0 ||
+        hipStreamSynchronize(f->stream) != hipSuccess ||
+        hipMemcpy(out, f->d_hout, sizeof(double) * nout * (size_t)f->nch, hipMemcpyDeviceToHost) != hipSuccess)
+        return -1;
+    return 0;
 }
 
 } /* extern "C" */
