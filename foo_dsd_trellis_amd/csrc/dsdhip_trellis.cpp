@@ -32,6 +32,9 @@ struct dsdhip_trellis {
     double kernel_ms;        /* accumulated */
     int64_t kernel_launches;
     int64_t samples;         /* inputs consumed by timed launches */
+    double *d_pin;           /* host-buffer entry: staged input / packed output, grow-only */
+    uint64_t *d_pout;
+    size_t pin_cap, pout_cap;
 };
 
 static void ctx_fresh(tr_ctx *c)
@@ -68,6 +71,9 @@ dsdhip_trellis_t *dsdhip_trellis_create_coeffs(const double *a, const double *g,
     t->snap_of.assign(nctx, -1);
     t->d_jobs = NULL;
     t->jobs_cap = 0;
+    t->d_pin = NULL;
+    t->d_pout = NULL;
+    t->pin_cap = t->pout_cap = 0;
     t->stream = 0;
     t->timing = 0;
     t->kernel_ms = 0.0;
@@ -147,6 +153,8 @@ void dsdhip_trellis_destroy(dsdhip_trellis_t *t)
     }
     hipFree(t->d_ctx);
     if (t->d_jobs) hipFree(t->d_jobs);
+    if (t->d_pin) hipFree(t->d_pin);
+    if (t->d_pout) hipFree(t->d_pout);
     delete t;
 }
 
@@ -372,11 +380,24 @@ int dsdhip_trellis_process(dsdhip_trellis_t *t, int i, const double *in, float *
                            size_t count, size_t *out_count)
 {
     if (!t || i < 0 || i >= t->nctx) return -1;
-    double *d_in = NULL;
-    uint64_t *d_out = NULL;
     size_t nw = (count + 63) / 64 + 1;
-    if (hipMalloc(&d_in, sizeof(double) * (count ? count : 1)) != hipSuccess) return -1;
-    if (hipMalloc(&d_out, sizeof(uint64_t) * nw) != hipSuccess) { hipFree(d_in); return -1; }
+    const size_t nin = count ? count : 1;
+    if (t->pin_cap < nin) {
+        if (t->d_pin) hipFree(t->d_pin);
+        t->d_pin = NULL;
+        t->pin_cap = 0;
+        if (hipMalloc(&t->d_pin, sizeof(double) * nin) != hipSuccess) return -1;
+        t->pin_cap = nin;
+    }
+    if (t->pout_cap < nw) {
+        if (t->d_pout) hipFree(t->d_pout);
+        t->d_pout = NULL;
+        t->pout_cap = 0;
+        if (hipMalloc(&t->d_pout, sizeof(uint64_t) * nw) != hipSuccess) return -1;
+        t->pout_cap = nw;
+    }
+    double *d_in = t->d_pin;
+    uint64_t *d_out = t->d_pout;
     int rc = -1;
     if (hipMemcpy(d_in, in, sizeof(double) * count, hipMemcpyHostToDevice) == hipSuccess) {
         const double *pin = d_in;
@@ -395,8 +416,6 @@ int dsdhip_trellis_process(dsdhip_trellis_t *t, int i, const double *in, float *
             }
         }
     }
-    hipFree(d_in);
-    hipFree(d_out);
     return rc;
 }
 
