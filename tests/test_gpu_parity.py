@@ -254,3 +254,19 @@ def test_das_reduces_to_sequential_reference(gpu, oracle, reference, nch, F, nse
         seq = reference.ctx(fi, 8, 8, 16).process(x[c])
         assert m == seq.size
         assert np.array_equal(y[c, :m], seq), c
+
+
+def test_das_chunks_of_changing_size(gpu, oracle):
+    """The host-buffer DAS entry over chunks that grow and shrink (its
+    device staging only grows) continues the persistent contexts exactly
+    like the oracle's chunked DAS."""
+    rate, K, L, B, nch, seg = 11289600, 16, 32, 8, 2, 16384
+    fi = oracle.ntf_auto(rate)
+    d = D.Das(fi, B, K, L, nch, seg_len=seg, seg0_first=False, reposition=True)
+    ref = ol.DasStream(oracle, fi, B, K, L, nch)
+    for k, F in enumerate([70000, 20000, 90000, 16384]):
+        x = np.stack([_x("pink", F, rate, 100 * k + c) for c in range(nch)])
+        y, m = d.run(x)
+        want = ref.chunk(x, seg, 0, 1)
+        assert m == want.shape[1], k
+        assert np.array_equal(y[:, :m], want), k
