@@ -5,15 +5,20 @@ Workload (one "step"): 30 s of stereo 44.1 kHz float32 PCM (seeded pink
 noise at -12 dBFS RMS clipped to +-0.5 plus 7 tones 100 Hz..15 kHz at
 -20 dBFS) -> eight 2x half-band FIR stages to DSD256 (11.2896 MHz, gain
 0.354) -> DAS parallel trellis (CLANS-8, K=32, L=64, B=8, 65536-sample
-segments, overlap 4096, all segments seeded from the persistent state,
-quiet-boundary reposition) -> stitched 1-bit streams.  Inputs are resident
-in HBM before the timed region; each step starts from fresh FIR delay lines
-and fresh trellis contexts.
+segments, overlap 4096, quiet-boundary reposition) -> stitched 1-bit
+streams.  The headline seeds as the reference does at DSD256 (seg0-first:
+seg0 on the persistent context, seg1 its continuation, the 16-level
+estimate from seg1's start seeding every later segment); the same step with
+every segment seeded from the persistent state (the reference's DSD512+
+mode) is timed beside it.  Inputs are resident in HBM before the timed
+region; each step starts from fresh FIR delay lines and fresh trellis
+contexts.
 
 Prints one JSON line: Msamples/s of DSD output over both channels, with
-x-realtime, the trellis kernel's roofline position (HIP-event timed), the
-plain per-channel GPU form on a bounded sample, and the CPU oracle timed on
-this host's cores.
+x-realtime, the trellis kernel's roofline position (HIP-event timed on the
+all-segments launch), the plain per-channel GPU form on a bounded sample,
+and the reference's DAS path (the oracle's C restatement, same plan, FIR
+included) timed on all of this host's cores.
 """
 import argparse
 import json
@@ -56,48 +61,87 @@ def traffic_per_input_sample():
         return None
 
 
-def cpu_baseline(x_seg, fi, threads, target_s):
-    """Trellis segments (K=32, L=64) on `threads` host threads: the
-    reference's own src/trellis.c compiled with gcc -O2 -ffp-contract=off
-    (oracle/_ref) where it was built, else the oracle's C restatement."""
+def host_cpu():
+    """Cores this process may use and the CPU model string."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return n, os.cpu_count(), model
+
+
+def cpu_baseline(pcm, seconds, seg0_first, threads):
+    """The reference's DAS path on host threads, bounded: the first `seconds`
+    of the headline's PCM through the oracle's FP64 FIR chain (one thread per
+    channel) and the oracle's DAS with the same plan (65,536-sample
+    segments, reposition, the given seeding; its phases run concurrently
+    over channels x segments on `threads` threads, as the reference's
+    threadpool does).  The oracle is the C restatement of src/fir.c,
+    src/trellis.c and src/dsp_plugin.c's DAS (gcc -O2 -ffp-contract=off),
+    bit-exact with the reference's own trellis.c; the reference's DAS
+    driver itself needs Win32 + IPP and does not build here."""
+    import ctypes as C
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib
-    O = oracle_lib.reference()
-    kind, what = "reference", "reference src/trellis.c (gcc -O2, scalar)"
-    if O is None:
-        O, kind, what = oracle_lib.oracle(), "port", "oracle C restatement"
-    name = oracle_lib.oracle().ntf_get(fi)["name"]
+    import oracle_lib as ol
+    from foo_dsd_trellis_amd import fir
+    O = ol.oracle()
+    n_in = int(seconds * FS_IN)
+    x = np.ascontiguousarray(pcm[:, :n_in])
     fi = O.ntf_auto(DSD256)
-    if O.ntf_get(fi)["name"] != name:
-        raise RuntimeError("CPU baseline NTF %s differs from %s" % (O.ntf_get(fi)["name"], name))
-    keep = SEG_LEN
-    # calibrate one segment on one thread
+    O.lib.ora_das_set_threads.argtypes = [C.c_int]
+    O.lib.ora_das_set_threads(threads)
     t0 = time.perf_counter()
-    O.ctx(fi, B, K, L).process(x_seg)
-    per = time.perf_counter() - t0
-    per_thread = max(1, int(round(target_s / per)))
-    done = []
+    ys = [None] * NCH
 
-    def work():
-        n = 0
-        for _ in range(per_thread):
-            out = O.ctx(fi, B, K, L).process(x_seg)
-            n += min(keep, out.size)
-        done.append(n)
+    def fir_ch(c):
+        ys[c] = ol.fir_chain(O, FS_IN, DSD256, x[c], fir.PCM_FIR_GAIN)
 
-    ts = [threading.Thread(target=work) for _ in range(threads)]
-    t0 = time.perf_counter()
-    for t in ts:
+    th = [threading.Thread(target=fir_ch, args=(c,)) for c in range(NCH)]
+    for t in th:
         t.start()
-    for t in ts:
+    for t in th:
         t.join()
-    wall = time.perf_counter() - t0
-    total = sum(done)
-    return {"value": round(total / wall / 1e6, 4), "unit": "Msamples/s", "cores": threads,
-            "kind": kind,
-            "sample": "%d threads x %d trellis segments of %d inputs (%d kept outputs each), "
-                      "CLANS-8 K=32 L=64 B=8, %s, %.1f s wall"
-                      % (threads, per_thread, x_seg.size, keep, what, wall)}
+    t1 = time.perf_counter()
+    out, n, sp, plan = ol.das(O, fi, B, K, L, np.stack(ys), seg_len=SEG_LEN, nseg=0,
+                              seg0_first=1 if seg0_first else 0, reposition=1)
+    t2 = time.perf_counter()
+    O.lib.ora_das_set_threads(1)
+    cores, nproc, model = host_cpu()
+    wall = t2 - t0
+    return {"value": round(NCH * n / wall / 1e6, 4), "unit": "Msamples/s",
+            "x_realtime": round(seconds / wall, 4), "cores": threads, "nproc": nproc,
+            "affinity_cores": cores, "cpu_model": model, "kind": "port",
+            "seed_mode": "seg0-first" if seg0_first else "all-parallel",
+            "fir_s": round(t1 - t0, 3), "das_s": round(t2 - t1, 3),
+            "sample": "first %.2f s of the headline PCM (%d DSD256 samples per channel, %d segments): "
+                      "oracle FP64 FIR chain (1 thread per channel) + oracle DAS, same plan, phases over "
+                      "%d threads, %.1f s wall" % (seconds, n_in * (DSD256 // FS_IN), len(plan), threads, wall)}
+
+
+def pmc_utilisation():
+    """VALU issue and LDS utilisation of the trellis launch from the SQ
+    counters committed in profiles/tr_run_jobs_dsd256_pmc.md (formulas
+    there)."""
+    import re
+    path = os.path.join(ROOT, "profiles", "tr_run_jobs_dsd256_pmc.md")
+    out = {}
+    try:
+        txt = open(path).read()
+    except OSError:
+        return out
+    for key in ("valu_issue_util", "lds_util", "fp64_valu_share"):
+        m = re.search(key + r":\s*([0-9.]+)", txt)
+        if m:
+            out[key] = float(m.group(1))
+    return out
 
 
 def diff_loop_filter(steps, check_rows):
@@ -259,9 +303,9 @@ def streaming_room(chunks, seg_len):
             "chunk_ms_p99": round(float(np.percentile(lat_ms, 99)), 3),
             "chunk_ms_max": round(float(lat_ms.max()), 3),
             "chunk_audio_ms": round(cf / FS_IN * 1000.0, 3), "timed_chunks": chunks, "untimed_chunks": warm,
-            "seg_len": seg_len,
+            "seg_len": seg_len, "seed_mode": "seg0-first" if S.seg0_first else "all-parallel",
             "config": "6 ch 44.1 kHz f32, 8192-frame chunks -> DSD256 (8 stages) -> UPOLS 2^20-tap "
-                      "room IR (P=8192, c-f32) -> DAS K=32 L=64 B=8, one stream, chunks in order"}
+                      "room IR (P=8192, c-f32) -> DAS K=32 L=64 B=8 seeded as the reference at DSD256, one stream, chunks in order"}
 
 
 def next_rows(d_bits, n_bits, seconds):
@@ -351,13 +395,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-seconds", type=float, default=1.0,
-                    help="seconds of CPU work per thread for the cpu_baseline leg: the reference trellis.c where built, else the oracle (0: skip)")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-audio-seconds", type=float, default=1.0,
+                    help="audio seconds of the headline workload for the cpu_baseline leg (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: every core this process may use")
     ap.add_argument("--plain-seconds", type=float, default=0.05,
                     help="audio seconds per channel for the plain per-channel GPU form (0: skip)")
     ap.add_argument("--seg-len", type=int, default=SEG_LEN)
-    ap.add_argument("--seg0-first", action="store_true")
+    ap.add_argument("--seed-mode", choices=["both", "seg0-first", "all-parallel"], default="both",
+                    help="headline seeding (seg0-first is the reference's at DSD256); both: "
+                         "all-parallel is timed beside it")
     ap.add_argument("--lf-steps", type=int, default=5,
                     help="differentiable loop filter fwd+bwd steps (0: skip)")
     ap.add_argument("--lf-check-rows", type=int, default=2)
@@ -391,38 +437,68 @@ def main():
     chain = fir.FirChain(FS_IN, DSD256, fir.PCM_FIR_GAIN, NCH)
     fi = trellis.ntf_auto(DSD256)
     P = trellis.ntf_row(fi)["order"]
-    D = das.Das(fi, B, K, L, NCH, seg_len=args.seg_len, seg0_first=args.seg0_first, reposition=True)
-    D.timing(True)
-
     fir_ev = []
 
-    def step(timed=False):
-        chain.reset()
-        D.reset()
-        if timed:
-            ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-            ev[0].record()
-        chain.process_dev(d_x.data_ptr(), n_in, n_in, d_f.data_ptr(), F)
-        if timed:
-            ev[1].record()
-            fir_ev.append(ev)
-        return D.run_dev(d_f.data_ptr(), F, F, d_bits.data_ptr(), words)
+    def run_mode(seg0_first):
+        D = das.Das(fi, B, K, L, NCH, seg_len=args.seg_len, seg0_first=seg0_first, reposition=True)
+        D.timing(True)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    D.timing_read(clear=True)
-    t0 = time.perf_counter()
-    n_out = 0
-    for _ in range(args.steps):
-        n_out = step(timed=True)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
+        def step(timed=False):
+            chain.reset()
+            D.reset()
+            if timed:
+                ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                ev[0].record()
+            chain.process_dev(d_x.data_ptr(), n_in, n_in, d_f.data_ptr(), F)
+            if timed:
+                ev[1].record()
+                fir_ev.append(ev)
+            return D.run_dev(d_f.data_ptr(), F, F, d_bits.data_ptr(), words)
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        D.timing_read(clear=True)
+        t0 = time.perf_counter()
+        n_out = 0
+        for _ in range(args.steps):
+            n_out = step(timed=True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        k_ms, k_launches, k_samples = D.timing_read(clear=True)
+        ms_step = (t1 - t0) * 1000.0 / args.steps
+        nseg, slow, _ = D.last_info(0)
+        r = {"value": round(NCH * n_out / (ms_step / 1000.0) / 1e6, 3), "unit": "Msamples/s",
+             "x_realtime": round(SECONDS / (ms_step / 1000.0), 3), "ms_per_step": round(ms_step, 3),
+             "out_samples_per_channel": n_out, "segments_per_channel": nseg, "literal_prune_steps": slow,
+             "trellis_ms_per_step": round(k_ms / args.steps, 3),
+             "trellis_launches_per_step": k_launches // args.steps,
+             "trellis_inputs_per_step": k_samples // args.steps}
+        if seg0_first:
+            est_ms, est_n, redo, nb = D.last_estimate()
+            r["estimator"] = {"host_chain_ms": round(est_ms, 3), "samples_all_channels": est_n,
+                              "ns_per_sample_per_chain": round(est_ms * 1e6 / max(1, est_n // NCH), 3),
+                              "reference_form_blocks": redo, "batches": nb,
+                              "note": "one host thread per channel; segments launched as their "
+                                      "estimates arrive, over 4 streams"}
+        return r, n_out
+
+    modes = ["seg0-first", "all-parallel"] if args.seed_mode == "both" else [args.seed_mode]
+    res = {}
+    for m in modes:
+        fir_ev.clear()
+        res[m], n_m = run_mode(m == "seg0-first")
+        if m == modes[0]:
+            n_out = n_m
+            fir_ms = sum(a.elapsed_time(b) for a, b in fir_ev) / max(1, len(fir_ev))
+            bits0 = d_bits.clone()
+    head = res[modes[0]]
+    value, ms_step, xrt = head["value"], head["ms_per_step"], head["x_realtime"]
+
     # FIR chain (8 half-band stages, one launch each): bytes = first-stage
     # float input + every stage's double output + the double inputs of
     # stages 2..8 (what a stage-per-launch chain must move); compulsory
     # bytes = the float input + the final double output only
-    fir_ms = sum(a.elapsed_time(b) for a, b in fir_ev) / max(1, len(fir_ev))
     nst = int(round(np.log2(DSD256 // FS_IN)))
     fir_bytes = NCH * n_in * (4 + 16 * (2 ** nst - 1) + 8 * (2 ** nst - 2))
     fir_min = NCH * n_in * (4 + 8 * 2 ** nst)
@@ -433,21 +509,28 @@ def main():
                 "compulsory_bytes": int(fir_min),
                 "frac_compulsory": round(fir_min / (fir_ms / 1e3) / 8e12, 4) if fir_ms > 0 else None,
                 "stages": nst, "note": "whole chain per step on the default stream, HIP events"}
-    k_ms, k_launches, k_samples = D.timing_read(clear=True)
-    ms_step = (t1 - t0) * 1000.0 / args.steps
-    total_out = NCH * n_out
-    value = total_out / (ms_step / 1000.0) / 1e6
-    xrt = SECONDS / (ms_step / 1000.0)
-    nseg, slow, _ = D.last_info(0)
 
+    # trellis roofline on the one-launch (all-segments) form: algorithmic
+    # FP64 flops = K (5P + 6) per output sample, outputs of both channels
     fps = flops_per_sample(P, K)
-    tb = traffic_per_input_sample()
-    achieved = (k_samples * fps) / (k_ms / 1000.0) / 1e12 if k_ms > 0 else 0.0
-    roof = {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 6),
-            "traffic": int(tb * k_samples / max(1, k_launches)) if tb else None,
-            "kernel": "tr_run_jobs<8,1>", "avg_launch_ms": round(k_ms / max(1, k_launches), 3),
-            "flops_per_input_sample": fps, "inputs_per_launch": k_samples // max(1, k_launches)}
+    roof = None
+    ap_ = res.get("all-parallel")
+    if ap_ is not None and ap_["trellis_launches_per_step"] == 1:
+        launch_ms = ap_["trellis_ms_per_step"]
+        outs = NCH * ap_["out_samples_per_channel"]
+        achieved = outs * fps / (launch_ms / 1000.0) / 1e12
+        tb = traffic_per_input_sample()
+        roof = {"bound": "valu", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 6),
+                "traffic": int(tb * ap_["trellis_inputs_per_step"]) if tb else None,
+                "kernel": "tr_run_jobs<8,1>", "avg_launch_ms": launch_ms,
+                "flops_per_output_sample": fps, "outputs_per_launch": outs,
+                "inputs_per_launch": ap_["trellis_inputs_per_step"],
+                "peak_note": "MI355X FP64 vector peak (no MFMA: the trellis is not a contraction)",
+                "measured_on": "all-parallel launch (every segment of both channels in one launch); "
+                               "the seg0-first headline is bound by its sequential estimate",
+                "traffic_source": "profiles/tr_run_jobs_dsd256_das_hbm_traffic.md (FETCH_SIZE/WRITE_SIZE)"}
+        roof.update(pmc_utilisation())
 
     plain = None
     if args.plain_seconds > 0:
@@ -467,29 +550,30 @@ def main():
                            "per channel (%d samples)" % (args.plain_seconds, npl)}
 
     cpu = None
-    if args.cpu_seconds > 0:
-        thr = max(1, min(args.cpu_threads, os.cpu_count() or 1))
-        seg = d_f[0, SEG_LEN:SEG_LEN + SEG_LEN + 2 * 64 * L].cpu().numpy()
-        cpu = cpu_baseline(np.ascontiguousarray(seg), fi, thr, args.cpu_seconds)
+    if args.cpu_audio_seconds > 0:
+        cores = host_cpu()[0]
+        thr = args.cpu_threads if args.cpu_threads > 0 else cores
+        cpu = cpu_baseline(x, args.cpu_audio_seconds, modes[0] == "seg0-first", thr)
 
     lf = diff_loop_filter(args.lf_steps, args.lf_check_rows) if args.lf_steps > 0 else None
     bu = (batched_uneven(args.batch_chunks, args.batch_chunk_frames, args.batch_plain_tracks)
           if args.batch_chunks > 1 else None)
     sr = streaming_room(args.stream_chunks, args.stream_seg_len) if args.stream_chunks > 0 else None
-    nr = next_rows(d_bits, n_out, args.next_seconds) if args.next_seconds > 0 else None
+    nr = next_rows(bits0, n_out, args.next_seconds) if args.next_seconds > 0 else None
 
     line = {
         "metric": "Trellis SDM Msamples/s & xrealtime at DSD256 K=32, batched vs plain",
-        "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True,
+        "value": value, "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "x_realtime": round(xrt, 3),
+        "x_realtime": xrt,
         "config": {"workload": "PCM 44.1k stereo 30 s -> 8x2 half-band FIR -> DSD256 DAS trellis",
                    "fs_out": DSD256, "seconds": SECONDS, "channels": NCH, "K": K, "L": L, "B": B,
                    "ntf": trellis.ntf_row(fi)["name"], "seg_len": args.seg_len,
-                   "overlap": 64 * L, "segments_per_channel": nseg,
-                   "seed_mode": "seg0-first" if args.seg0_first else "all-parallel",
-                   "literal_prune_steps": slow, "out_samples_per_channel": n_out},
+                   "overlap": 64 * L, "segments_per_channel": head["segments_per_channel"],
+                   "seed_mode": modes[0],
+                   "literal_prune_steps": head["literal_prune_steps"], "out_samples_per_channel": n_out},
+        "seed_modes": res,
         "roofline": roof,
         "fir_roofline": fir_roof,
         "plain_gpu": plain,

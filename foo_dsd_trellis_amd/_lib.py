@@ -49,6 +49,15 @@ _SIGS = {
     "dsdhip_trellis_drain": ([_vp, C.c_int, _fp, C.c_size_t, C.POINTER(C.c_size_t)], C.c_int),
     "dsdhip_trellis_state_distance": ([_vp, C.c_int, C.c_int, _dp], C.c_int),
     "dsdhip_estimate_state": ([_dp, _dp, C.c_int, _dp, _dp, C.c_size_t, C.c_double, C.c_int, _dp], C.c_int),
+    "dsdhip_estimate_boundaries": ([_dp, _dp, C.c_int, _dp, _dp, C.c_size_t, _i64p, C.c_int, C.c_double,
+                                    C.c_int, _dp], C.c_int),
+    "dsdhip_estimate_boundaries_ex": ([_dp, _dp, C.c_int, _dp, _dp, C.c_size_t, _i64p, C.c_int, C.c_double,
+                                       C.c_int, _dp, C.c_double, _i64p], C.c_int),
+    "dsdhip_estimate_boundaries_dev": ([_dp, _dp, C.c_int, _dp, _vp, C.c_int64, C.c_int, C.c_size_t, _i64p,
+                                        C.c_int, C.c_double, C.c_int, _dp], C.c_int),
+    "dsdhip_est_table_index": ([C.c_double, C.c_int, C.c_double], C.c_int),
+    "dsdhip_das_estimator_device": ([_vp, C.c_int], C.c_int),
+    "dsdhip_das_last_estimate": ([_vp, _dp, _i64p, _i64p, _ip], C.c_int),
     "dsdhip_das_wave_only": ([_vp, C.c_int], C.c_int),
     "gpu_hip_fir_lowpass_setup": ([_vp, _fp, C.c_int], C.c_int),
     "gpu_hip_fir_lowpass_f64": ([_vp, _fp, _dp, C.c_size_t, C.c_double], C.c_int),
@@ -90,6 +99,9 @@ _SIGS = {
     "gpu_hip_trellis": ([_vp, _dp, _fp, C.c_size_t], C.c_int),
     "gpu_hip_trellis_das": ([_vp, _dp, _fp, C.c_size_t, C.c_int], C.c_int),
     "gpu_hip_das_config": ([_vp, C.c_size_t, C.c_int, C.c_int], C.c_int),
+    "gpu_hip_das_rate": ([_vp, C.c_uint], C.c_int),
+    "gpu_hip_trellis_das_ex": ([_vp, _dp, _fp, C.c_size_t, C.c_int, C.POINTER(C.c_size_t)], C.c_int),
+    "gpu_hip_trellis_das_count": ([_vp], C.c_size_t),
     "dsdhip_lf_checkpoints": ([C.c_size_t, C.c_int], C.c_size_t),
     "dsdhip_lf_workspace_bytes": ([C.c_int, C.c_size_t, C.c_int, C.c_int, C.c_int], C.c_size_t),
     "dsdhip_lf_forward": ([_vp, _vp, C.c_int, C.c_double, _vp, C.c_int, C.c_size_t, C.c_int, _vp,

@@ -489,8 +489,10 @@ static int host_setup(gpu_hip_plan_t *p)
     p->fir = dsdhip_fir_create(p->fs_in, p->fs_out, p->gain, ns);
     if (!p->fir) return -1;
     dsdhip_fir_set_stream(p->fir, p->stream);
+    /* seeding by the reference's rule for the output rate (src/dsp_plugin.c:2185) */
     p->das = dsdhip_das_create_tracks(p->ntf, p->depth, p->K, p->L, p->state_limit, p->nch,
-                                      p->ntracks, (size_t)p->tune.seg_len, 0, 0, p->reposition);
+                                      p->ntracks, (size_t)p->tune.seg_len, 0,
+                                      p->fs_out < DSDHIP_DSD512_RATE ? 1 : 0, p->reposition);
     if (!p->das) return -1;
     dsdhip_das_set_stream(p->das, p->stream);
     p->tmp_words = Ffull / 64 + 2;

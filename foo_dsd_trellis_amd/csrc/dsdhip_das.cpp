@@ -24,6 +24,10 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <thread>
 #include <vector>
 #include "dsdhip_internal.h"
 
@@ -71,6 +75,9 @@ struct DevBuf {
 
 /* per-object device scratch of a run, kept between calls (grow-only); like
  * the trellis contexts, one object is used from one thread at a time */
+/* seg0-first estimate input staged in pinned host memory, copied in blocks
+ * of EST_BLOCK samples with an event after each */
+#define EST_BLOCK ((size_t)1 << 22)
 struct DasScratch {
     DevBuf<int64_t> dc, db, dpb, dcb;
     DevBuf<double> ds, dinit, dest, dab, dmx;
@@ -80,6 +87,53 @@ struct DasScratch {
     DevBuf<double> hin;                           /* host-buffer entry: staged input */
     DevBuf<unsigned long long> hbits;             /* ... packed output */
     DevBuf<float> hout;                           /* ... +-1.0 output */
+    /* seg0-first pipeline: the batch stream, batch completion events, the
+     * estimate input staged in pinned host memory and its block events */
+    hipStream_t bs = 0;
+    std::vector<hipEvent_t> bev, blk;
+    hipEvent_t ev_go = 0, ev_t0 = 0, ev_t1 = 0;
+    double *h_est = nullptr;
+    size_t h_est_cap = 0;
+    int pipe_ok = 0;
+    int init_pipe()
+    {
+        if (pipe_ok) return 0;
+        if (hipStreamCreateWithFlags(&bs, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&ev_go, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreate(&ev_t0) != hipSuccess || hipEventCreate(&ev_t1) != hipSuccess)
+            return -1;
+        pipe_ok = 1;
+        return 0;
+    }
+    static int grow_events(std::vector<hipEvent_t> &v, size_t n)
+    {
+        while (v.size() < n) {
+            hipEvent_t e;
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return -1;
+            v.push_back(e);
+        }
+        return 0;
+    }
+    int ensure_host(size_t n)
+    {
+        if (n <= h_est_cap) return 0;
+        if (h_est) hipHostFree(h_est);
+        h_est = nullptr;
+        h_est_cap = 0;
+        if (hipHostMalloc((void **)&h_est, sizeof(double) * n, 0) != hipSuccess) return -1;
+        h_est_cap = n;
+        return 0;
+    }
+    ~DasScratch()
+    {
+        for (hipEvent_t e : bev) hipEventDestroy(e);
+        for (hipEvent_t e : blk) hipEventDestroy(e);
+        if (bs) hipStreamDestroy(bs);
+        if (ev_go) hipEventDestroy(ev_go);
+        if (ev_t0) hipEventDestroy(ev_t0);
+        if (ev_t1) hipEventDestroy(ev_t1);
+        if (h_est) hipHostFree(h_est);
+    }
 };
 
 struct dsdhip_das {
@@ -96,10 +150,15 @@ struct dsdhip_das {
     hipStream_t stream;
     double qfir[255];
     double *d_qfir;
+    int est_device;              /* seg0-first estimate on one device lane per channel (timing) */
     /* last run */
     int last_S;
     std::vector<int> last_stitch;
     int64_t last_slow;
+    double last_est_ms;          /* estimator threads: longest busy time */
+    int64_t last_est_samples;    /* samples estimated (all channels) */
+    int64_t last_est_redo;       /* estimator blocks recomputed in the reference form */
+    int last_batches;            /* pipelined trellis launches */
     DasScratch sc;
 };
 
@@ -175,6 +234,11 @@ static dsdhip_das_t *das_create(const double *a, const double *g, int order, int
     d->d_qfir = NULL;
     d->last_S = 0;
     d->last_slow = 0;
+    d->est_device = 0;
+    d->last_est_ms = 0.0;
+    d->last_est_samples = 0;
+    d->last_est_redo = 0;
+    d->last_batches = 0;
     dsdhip_make_qfir(d->qfir);
     if (dsdhip_ensure_device() != 0 ||
         hipMalloc(&d->d_qfir, sizeof(d->qfir)) != hipSuccess ||
@@ -240,43 +304,52 @@ int dsdhip_das_wave_only(dsdhip_das_t *d, int on) { return d ? dsdhip_trellis_wa
 
 /* sdm_estimate_state_multibit (src/trellis.c:1118-1187): the greedy
  * num_levels quantizer run through the loop filter from init_state over
- * count host samples, on one device lane (the DAS seeding kernel). */
+ * count host samples, on the calling thread (dsdhip_estimate.cpp). */
 int dsdhip_estimate_state(const double *a, const double *g, int order, const double *init_state,
                           const double *in, size_t count, double state_limit, int num_levels,
                           double *out_state)
 {
-    if (!a || !g || !init_state || !out_state || (!in && count) || order < 2 || order > TR_MAXP ||
+    if (!a || !g || !init_state || !out_state || (!in && count) || order < 4 || order > TR_MAXP ||
         num_levels < 2)
         return -1;
-    if (dsdhip_ensure_device() != 0) return -1;
-    double init[10] = {0}, ab[2 * TR_MAXP] = {0}, est[10];
+    double init[10] = {0}, est[10];
     memcpy(init, init_state, sizeof(double) * (size_t)order);
+    const int64_t bnd = (int64_t)count;
+    if (dsdhip_estimate_boundaries(a, g, order, init, in, count, &bnd, 1, state_limit, num_levels, est) != 0)
+        return -1;
+    memcpy(out_state, est, sizeof(double) * (size_t)order);
+    return 0;
+}
+
+/* The estimation block on one device lane per channel (das_estimate), for
+ * timing against the host form.  d_x: nch rows of stride doubles. */
+int dsdhip_estimate_boundaries_dev(const double *a, const double *g, int order, const double *init10,
+                                   const double *d_x, int64_t stride, int nch, size_t count,
+                                   const int64_t *bnd, int nb, double state_limit, int levels,
+                                   double *out10)
+{
+    if (!a || !g || !init10 || !d_x || nch < 1 || nb < 1 || !bnd || !out10 || order < 4 ||
+        order > TR_MAXP || levels < 2)
+        return -1;
+    if (dsdhip_ensure_device() != 0) return -1;
+    double ab[2 * TR_MAXP] = {0};
     memcpy(ab, a, sizeof(double) * (size_t)order);
     memcpy(ab + TR_MAXP, g, sizeof(double) * (size_t)order);
-    const int64_t bnd = (int64_t)count;
-    double *dx = NULL, *dinit = NULL, *dab = NULL, *dest = NULL;
-    int64_t *db = NULL;
-    int rc = -1;
-    if (hipMalloc(&dx, sizeof(double) * (count ? count : 1)) == hipSuccess &&
-        hipMalloc(&dinit, sizeof(init)) == hipSuccess && hipMalloc(&dab, sizeof(ab)) == hipSuccess &&
-        hipMalloc(&dest, sizeof(est)) == hipSuccess && hipMalloc(&db, sizeof(bnd)) == hipSuccess &&
-        (count == 0 || hipMemcpy(dx, in, sizeof(double) * count, hipMemcpyHostToDevice) == hipSuccess) &&
-        hipMemcpy(dinit, init, sizeof(init), hipMemcpyHostToDevice) == hipSuccess &&
-        hipMemcpy(dab, ab, sizeof(ab), hipMemcpyHostToDevice) == hipSuccess &&
-        hipMemcpy(db, &bnd, sizeof(bnd), hipMemcpyHostToDevice) == hipSuccess) {
-        hipLaunchKernelGGL(das_estimate, dim3(1), dim3(64), 0, 0, dx, (int64_t)count, (int64_t)0, (int64_t)count,
-                           db, 1, dinit, dab, dab + TR_MAXP, order, state_limit, num_levels, 1, dest);
-        if (hipGetLastError() == hipSuccess && hipMemcpy(est, dest, sizeof(est), hipMemcpyDeviceToHost) == hipSuccess) {
-            memcpy(out_state, est, sizeof(double) * (size_t)order);
-            rc = 0;
-        }
-    }
-    hipFree(dx);
-    hipFree(dinit);
-    hipFree(dab);
-    hipFree(dest);
-    hipFree(db);
-    return rc;
+    DevBuf<double> dinit, dab, dest;
+    DevBuf<int64_t> db;
+    if (dinit.ensure((size_t)nch * 10) || dab.ensure(2 * TR_MAXP) || dest.ensure((size_t)nch * nb * 10) ||
+        db.ensure((size_t)nb) ||
+        hipMemcpy(dinit.p, init10, sizeof(double) * (size_t)nch * 10, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dab.p, ab, sizeof(ab), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(db.p, bnd, sizeof(int64_t) * (size_t)nb, hipMemcpyHostToDevice) != hipSuccess)
+        return -1;
+    hipLaunchKernelGGL(das_estimate, dim3((unsigned)((nch + 63) / 64)), dim3(64), 0, 0, d_x, stride,
+                       (int64_t)0, (int64_t)count, db.p, nb, dinit.p, dab.p, dab.p + TR_MAXP, order,
+                       state_limit, levels, nch, dest.p);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpy(out10, dest.p, sizeof(double) * (size_t)nch * nb * 10, hipMemcpyDeviceToHost) != hipSuccess)
+        return -1;
+    return 0;
 }
 
 /* Plan for F samples: returns the segment count; fills start/size if given. */
@@ -477,14 +550,253 @@ struct TrackRun {
     std::vector<int64_t> kept, slow;               /* per (ch, seg) */
     std::vector<int> bp;                           /* stitch position per segment */
     int64_t total = 0;
+    const double *hin = nullptr;                   /* host copy of the input rows, or null */
 };
+
+/* One channel's estimator chain: from seg1's start over the rest of the
+ * chunk, the state at the start of every segment >= 2. */
+struct EstChain {
+    int t = 0, ch = 0;
+    const double *dx = nullptr;    /* device input at seg1's start */
+    const double *hx = nullptr;    /* host input at seg1's start (caller's or staged) */
+    const hipEvent_t *blk = nullptr;   /* staged: event of each EST_BLOCK block, or null */
+    size_t count = 0;
+    std::vector<int64_t> bnd;      /* offsets of segments 2.. from seg1's start */
+    double init[10] = {};
+    std::vector<double> est;       /* [nb][10] */
+    std::atomic<int> ready{0};     /* boundaries whose state is in est */
+    double busy_ms = 0.0;
+    int64_t redo = 0;
+};
+
+/* advances e over [pos, end) of chain c, waiting for each staged block */
+static int est_feed(dsdhip_est_t *e, const EstChain *c, size_t pos, size_t end)
+{
+    while (pos < end) {
+        const size_t b = pos / EST_BLOCK;
+        const size_t bend = std::min((b + 1) * EST_BLOCK, end);
+        if (c->blk && hipEventSynchronize(c->blk[b]) != hipSuccess) return -1;
+        dsdhip_est_feed(e, c->hx + pos, bend - pos);
+        pos = bend;
+    }
+    return 0;
+}
+
+static void est_worker(const dsdhip_das_t *d, std::vector<EstChain *> chains, std::atomic<int> *fail)
+{
+    for (EstChain *c : chains) {
+        const int nb = (int)c->bnd.size();
+        const auto t0 = std::chrono::steady_clock::now();
+        dsdhip_est_t *e = dsdhip_est_create(d->a, d->g, d->order, d->state_limit, 16, c->init);
+        bool ok = e != nullptr;
+        size_t pos = 0;
+        for (int b = 0; b < nb; b++) {
+            const int64_t bd = c->bnd[(size_t)b];
+            if (ok && bd > (int64_t)pos && (size_t)bd <= c->count) {
+                ok = est_feed(e, c, pos, (size_t)bd) == 0;
+                pos = (size_t)bd;
+            }
+            if (ok) dsdhip_est_get_state(e, &c->est[(size_t)b * 10]);
+            c->ready.store(b + 1, std::memory_order_release);
+        }
+        if (!ok) fail->store(1);
+        if (e) {
+            c->redo = dsdhip_est_redo_blocks(e);
+            dsdhip_est_destroy(e);
+        }
+        c->busy_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+}
+
+/*
+ * seg0-first seeding after seg0 has run (src/dsp_plugin.c:2185-2266, 2288-
+ * 2363): seg1 continues from the persistent context at once; the estimate
+ * from seg1's start runs on host threads (one chain per channel, reading
+ * the input staged to pinned host memory block by block) and the segments
+ * >= 2 are launched as their estimates arrive: a batch of everything ready
+ * whenever the previous batch has finished (the launches run in order on
+ * one stream), so the work left after the estimate ends is one batch.
+ * Fills kept/slow of every segment >= 1.
+ */
+static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned long long *segbits)
+{
+    const int nch = d->nch, T = d->ntracks;
+    dsdhip_trellis_t *tr = d->tr;
+    DasScratch &sc = d->sc;
+    if (sc.init_pipe() != 0) return -1;
+    struct Ent { int t, ch, seg; };
+    std::vector<Ent> ent;
+    size_t nchains = 0;
+    for (int t = 0; t < T; t++)
+        if (R[(size_t)t].S > 2) nchains += (size_t)nch;
+    std::vector<EstChain> chains(nchains);
+    int total = 0;
+    size_t ci = 0, staged = 0, nblk = 0;
+    for (int t = 0; t < T; t++) {
+        TrackRun &r = R[(size_t)t];
+        if (r.S >= 2) total += nch * (r.S - 1);
+        if (r.S <= 2) continue;
+        const int64_t es = (int64_t)r.st[1];
+        for (int ch = 0; ch < nch; ch++, ci++) {
+            EstChain &c = chains[ci];
+            c.t = t;
+            c.ch = ch;
+            c.dx = r.in + (int64_t)ch * r.in_stride + es;
+            c.hx = r.hin ? r.hin + (size_t)ch * r.F + (size_t)es : nullptr;
+            c.count = r.F - (size_t)es;
+            if (!c.hx) {
+                staged += c.count;
+                nblk += (c.count + EST_BLOCK - 1) / EST_BLOCK;
+            }
+            for (int seg = 2; seg < r.S; seg++) c.bnd.push_back((int64_t)r.st[(size_t)seg] - es);
+            c.est.assign(c.bnd.size() * 10, 0.0);
+            if (dsdhip_trellis_best_state(tr, ix_persist(d, t * nch + ch), c.init) != 0) return -1;
+        }
+    }
+    d->last_batches = 0;
+    d->last_est_ms = 0.0;
+    d->last_est_samples = 0;
+    d->last_est_redo = 0;
+    if (total == 0) return 0;
+    if (dsdhip_trellis_table_begin(tr, total) != 0) return -1;
+    if (staged && (sc.ensure_host(staged) != 0 || DasScratch::grow_events(sc.blk, nblk) != 0)) return -1;
+    if (DasScratch::grow_events(sc.bev, (size_t)total + 1) != 0) return -1;
+    ent.reserve((size_t)total);
+    auto fill = [&](int t, int ch, int seg, const double *seed) -> int {
+        TrackRun &r = R[(size_t)t];
+        const int sidx = t * nch + ch;
+        const int j = (int)ent.size();
+        ent.push_back({t, ch, seg});
+        return dsdhip_trellis_table_fill(tr, j, ix_seg(d, sidx, seg), ix_persist(d, sidx),
+                                         r.in + (int64_t)ch * r.in_stride + r.in_start[(size_t)seg],
+                                         r.in_cnt[(size_t)seg], r.keep[(size_t)seg],
+                                         (uint64_t *)(segbits + r.woff[(size_t)(ch * r.S + seg)]), seed);
+    };
+    /* the batch stream starts after seg0 and all earlier work */
+    const bool timing = d->timing != 0;
+    if (hipEventRecord(sc.ev_go, d->stream) != hipSuccess) return -1;
+    if (timing) hipEventRecord(sc.ev_t0, d->stream);
+    if (hipStreamWaitEvent(sc.bs, sc.ev_go, 0) != hipSuccess) return -1;
+    /* stage the estimate input: every copy is queued now, block i of every
+     * chain before block i + 1 of any, each followed by its event */
+    {
+        size_t off = 0, ei = 0;
+        std::vector<size_t> base(nchains, 0), first_ev(nchains, 0);
+        for (size_t i = 0; i < nchains; i++) {
+            if (chains[i].hx) continue;
+            base[i] = off;
+            off += chains[i].count;
+            first_ev[i] = ei;
+            ei += (chains[i].count + EST_BLOCK - 1) / EST_BLOCK;
+            chains[i].hx = sc.h_est + base[i];
+            chains[i].blk = &sc.blk[first_ev[i]];
+        }
+        for (size_t b = 0;; b++) {
+            bool any = false;
+            for (size_t i = 0; i < nchains; i++) {
+                EstChain &c = chains[i];
+                if (!c.blk || b * EST_BLOCK >= c.count) continue;
+                any = true;
+                const size_t n = std::min(EST_BLOCK, c.count - b * EST_BLOCK);
+                if (hipMemcpyAsync(sc.h_est + base[i] + b * EST_BLOCK, c.dx + b * EST_BLOCK, sizeof(double) * n,
+                                   hipMemcpyDeviceToHost, d->stream) != hipSuccess ||
+                    hipEventRecord(sc.blk[first_ev[i] + b], d->stream) != hipSuccess)
+                    return -1;
+            }
+            if (!any) break;
+        }
+    }
+    int launched = 0, nbatch = 0;
+    auto launch = [&]() -> int {
+        const int n = (int)ent.size() - launched;
+        if (n <= 0) return 0;
+        int rc = dsdhip_trellis_table_launch(tr, launched, n, sc.bs);
+        if (rc == 0 && hipEventRecord(sc.bev[(size_t)nbatch], sc.bs) != hipSuccess) rc = -1;
+        launched += n;
+        nbatch++;
+        return rc;
+    };
+    auto idle = [&]() -> bool {
+        return nbatch == 0 || hipEventQuery(sc.bev[(size_t)nbatch - 1]) == hipSuccess;
+    };
+    /* seg1: exact continuation, no estimate needed */
+    int rc = 0;
+    for (int t = 0; t < T && rc == 0; t++)
+        for (int ch = 0; ch < nch && R[(size_t)t].S >= 2 && rc == 0; ch++) rc = fill(t, ch, 1, NULL);
+    if (rc == 0) rc = launch();
+    if (nchains > 0 && rc == 0) {
+        /* estimator threads: chains dealt round-robin over at most 16
+         * threads, one core left to this one */
+        const unsigned hw = std::thread::hardware_concurrency();
+        const int nw = (int)std::min<size_t>(nchains, std::max(1u, std::min(hw > 1 ? hw - 1 : 1, 16u)));
+        std::vector<std::vector<EstChain *>> part((size_t)nw);
+        for (size_t i = 0; i < nchains; i++) part[i % (size_t)nw].push_back(&chains[i]);
+        std::atomic<int> fail{0};
+        std::vector<std::thread> th;
+        for (int i = 0; i < nw; i++) th.emplace_back(est_worker, d, part[(size_t)i], &fail);
+        std::vector<int> taken(nchains, 0);
+        for (;;) {
+            bool all = true;
+            for (size_t i = 0; i < nchains && rc == 0; i++) {
+                EstChain &c = chains[i];
+                const int nb = (int)c.bnd.size();
+                const int rd = c.ready.load(std::memory_order_acquire);
+                for (int b = taken[i]; b < rd && rc == 0; b++)
+                    rc = fill(c.t, c.ch, b + 2, &c.est[(size_t)b * 10]);
+                taken[i] = rd;
+                if (rd < nb) all = false;
+            }
+            if (rc != 0 || fail.load()) break;
+            const int pending = (int)ent.size() - launched;
+            if (pending > 0 && (all || idle())) {
+                rc = launch();
+                if (rc != 0) break;
+            }
+            if (all) break;
+            std::this_thread::sleep_for(std::chrono::microseconds(200));
+        }
+        for (auto &x : th) x.join();
+        if (fail.load()) rc = -1;
+        for (size_t i = 0; i < nchains; i++) {
+            d->last_est_ms = std::max(d->last_est_ms, chains[i].busy_ms);
+            d->last_est_samples += (int64_t)chains[i].count;
+            d->last_est_redo += chains[i].redo;
+        }
+    }
+    /* join the batch stream back into the DAS stream, then read results */
+    if (hipEventRecord(sc.bev[(size_t)nbatch], sc.bs) != hipSuccess ||
+        hipStreamWaitEvent(d->stream, sc.bev[(size_t)nbatch], 0) != hipSuccess)
+        rc = -1;
+    if (rc != 0) {
+        hipStreamSynchronize(sc.bs);
+        hipStreamSynchronize(d->stream);
+        return -1;
+    }
+    if (launched != total) return -1;
+    if (timing) hipEventRecord(sc.ev_t1, d->stream);
+    std::vector<int64_t> kp((size_t)total), sl((size_t)total);
+    int err = 0;
+    if (dsdhip_trellis_table_end(tr, d->stream, kp.data(), sl.data(), &err) != 0) return -1;
+    if (timing) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, sc.ev_t0, sc.ev_t1) == hipSuccess) dsdhip_trellis_timing_add(tr, ms, nbatch);
+    }
+    for (int j = 0; j < total; j++) {
+        const Ent &q = ent[(size_t)j];
+        TrackRun &r = R[(size_t)q.t];
+        r.kept[(size_t)(q.ch * r.S + q.seg)] = kp[(size_t)j];
+        r.slow[(size_t)(q.ch * r.S + q.seg)] = sl[(size_t)j];
+    }
+    d->last_batches = nbatch;
+    return 0;
+}
 
 /* DAS over every track (stream s = t * nch + ch) with all segments of all
  * tracks in one trellis launch (two with seg0-first seeding), all
  * boundaries in one stitch launch.  Tracks with F[t] == 0 are left alone. */
 static int run_tracks(dsdhip_das_t *d, const double *const *d_in, const int64_t *in_stride,
                       const size_t *F, unsigned long long *const *d_out, const int64_t *out_words,
-                      size_t *out_count)
+                      size_t *out_count, const double *const *h_in)
 {
     const int nch = d->nch, T = d->ntracks;
     const size_t O = d->O;
@@ -497,6 +809,7 @@ static int run_tracks(dsdhip_das_t *d, const double *const *d_in, const int64_t 
         if (F[t] == 0) continue;
         if (!d_in[t] || !d_out[t]) return -1;
         r.in = d_in[t];
+        r.hin = h_in ? h_in[t] : nullptr;
         r.in_stride = in_stride[t];
         r.F = F[t];
         r.out = d_out[t];
@@ -603,6 +916,10 @@ static int run_tracks(dsdhip_das_t *d, const double *const *d_in, const int64_t 
         for (int t = 0; t < T; t++)
             for (int ch = 0; ch < nch && R[(size_t)t].S > 0; ch++) add(t, ch, 0, -1, NULL);
         if (run() != 0) return -1;
+        if (!d->est_device) {
+            if (seg0_first_rest(d, R, segbits.p) != 0) return -1;
+        } else {
+        /* the same estimate on one device lane per channel (for timing) */
         std::vector<std::vector<double>> est((size_t)T);
         tr_params prm;
         dsdhip_trellis_params(tr, &prm);
@@ -656,6 +973,7 @@ static int run_tracks(dsdhip_das_t *d, const double *const *d_in, const int64_t 
             }
         }
         if (run() != 0) return -1;
+        }
     } else {
         std::vector<double> pst((size_t)d->ns * 10);
         for (int t = 0; t < T; t++)
@@ -835,7 +1153,7 @@ int dsdhip_das_run_dev(dsdhip_das_t *d, const double *d_in, int64_t in_stride, s
                        unsigned long long *d_out, int64_t out_words, size_t *out_count)
 {
     if (!d || !d_in || !d_out || F == 0 || d->ntracks != 1) return -1;
-    return run_tracks(d, &d_in, &in_stride, &F, &d_out, &out_words, out_count);
+    return run_tracks(d, &d_in, &in_stride, &F, &d_out, &out_words, out_count, NULL);
 }
 
 /* Every track of a multi-track object: d_in[t] = [nch][in_stride[t]] doubles
@@ -846,7 +1164,7 @@ int dsdhip_das_run_tracks_dev(dsdhip_das_t *d, const double *const *d_in, const 
                               const int64_t *out_words, size_t *out_count)
 {
     if (!d || !d_in || !in_stride || !F || !d_out || !out_words) return -1;
-    return run_tracks(d, d_in, in_stride, F, d_out, out_words, out_count);
+    return run_tracks(d, d_in, in_stride, F, d_out, out_words, out_count, NULL);
 }
 
 int dsdhip_das_last_info(const dsdhip_das_t *d, int *nseg, int64_t *slow_steps, int *stitch_pos,
@@ -860,7 +1178,9 @@ int dsdhip_das_last_info(const dsdhip_das_t *d, int *nseg, int64_t *slow_steps, 
     return 0;
 }
 
-/* Host-buffer form: in nch x F doubles, out nch x F floats (+-1.0). */
+/* Host-buffer form: in nch x F doubles, out nch x F floats.  Row c gets
+ * the *out_count produced samples (+-1.0) followed by 0.0f up to F (a value
+ * no DSD sample takes), so no earlier call's output shows through. */
 int dsdhip_das_run(dsdhip_das_t *d, const double *in, size_t F, float *out, size_t *out_count)
 {
     if (!d) return -1;
@@ -871,15 +1191,47 @@ int dsdhip_das_run(dsdhip_das_t *d, const double *in, size_t F, float *out, size
     size_t cnt = 0;
     if (sc.hin.ensure(F * (size_t)nch) || sc.hbits.ensure((size_t)words * (size_t)nch) ||
         sc.hout.ensure(F * (size_t)nch) ||
-        hipMemcpy(sc.hin.p, in, sizeof(double) * F * (size_t)nch, hipMemcpyHostToDevice) != hipSuccess ||
-        dsdhip_das_run_dev(d, sc.hin.p, (int64_t)F, F, sc.hbits.p, words, &cnt) != 0)
+        hipMemcpy(sc.hin.p, in, sizeof(double) * F * (size_t)nch, hipMemcpyHostToDevice) != hipSuccess)
         return -1;
-    hipLaunchKernelGGL(bits_to_f32, dim3((unsigned)((cnt + 255) / 256 + 1), (unsigned)nch), dim3(256), 0,
-                       d->stream, sc.hbits.p, words, (int64_t)cnt, sc.hout.p, (int64_t)F);
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(d->stream) != hipSuccess ||
-        hipMemcpy(out, sc.hout.p, sizeof(float) * F * (size_t)nch, hipMemcpyDeviceToHost) != hipSuccess)
-        return -1;
+    /* the estimator threads read the caller's host rows directly */
+    const double *din = sc.hin.p, *hin = in;
+    int64_t stride = (int64_t)F;
+    if (run_tracks(d, &din, &stride, &F, &sc.hbits.p, &words, &cnt, &hin) != 0) return -1;
+    if (cnt > F) return -1;
+    if (cnt > 0) {
+        hipLaunchKernelGGL(bits_to_f32, dim3((unsigned)((cnt + 255) / 256 + 1), (unsigned)nch), dim3(256), 0,
+                           d->stream, sc.hbits.p, words, (int64_t)cnt, sc.hout.p, (int64_t)F);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(d->stream) != hipSuccess ||
+            hipMemcpy2D(out, sizeof(float) * F, sc.hout.p, sizeof(float) * F, sizeof(float) * cnt, (size_t)nch,
+                        hipMemcpyDeviceToHost) != hipSuccess)
+            return -1;
+    }
+    for (int c = 0; c < nch; c++)
+        for (size_t i = cnt; i < F; i++) out[(size_t)c * F + i] = 0.0f;
     if (out_count) *out_count = cnt;
+    return 0;
+}
+
+/* Seeding estimate on one device lane per channel instead of host threads
+ * (same states; for timing the two against each other). */
+int dsdhip_das_estimator_device(dsdhip_das_t *d, int on)
+{
+    if (!d) return -1;
+    d->est_device = on ? 1 : 0;
+    return 0;
+}
+
+/* Estimator figures of the last seg0-first run: longest chain busy time,
+ * samples estimated over all channels, blocks recomputed in the reference
+ * form, pipelined trellis launches. */
+int dsdhip_das_last_estimate(const dsdhip_das_t *d, double *busy_ms, int64_t *samples, int64_t *redo_blocks,
+                             int *batches)
+{
+    if (!d) return -1;
+    if (busy_ms) *busy_ms = d->last_est_ms;
+    if (samples) *samples = d->last_est_samples;
+    if (redo_blocks) *redo_blocks = d->last_est_redo;
+    if (batches) *batches = d->last_batches;
     return 0;
 }
 

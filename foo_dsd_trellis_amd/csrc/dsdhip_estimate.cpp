@@ -1,0 +1,319 @@
+/*
+ * DAS state estimator on host threads: the 16-level greedy loop-filter
+ * recurrence of sdm_estimate_state_multibit (src/trellis.c:1118-1187), run
+ * from seg1's start across every later segment boundary as the estimation
+ * block of src/threadpool.c:345-366 does.
+ *
+ * Why the host: the recurrence is one dependent chain per channel (the
+ * state after sample n feeds sample n+1 through the quantizer), so its
+ * speed is the latency of that chain.  A GPU lane issues a dependent FP64
+ * operation every few cycles at 2.4 GHz and the reference's form divides
+ * twice per sample; one host core runs the same chain several times faster.
+ *
+ * Bit-exact with the reference, faster per sample:
+ *   - the quantizer idx = clamp((int)((v / a0 + 1) / step + 0.5)) is a
+ *     non-decreasing step function of v (for a0 > 0) up to the point where
+ *     the int conversion overflows to INT_MIN (idx 0).  Its exact
+ *     transition points are found once by bisection over the ordered
+ *     doubles, evaluating the reference formula itself, so idx k holds
+ *     exactly for v in [lo[k], hi[k]);
+ *   - on the per-sample chain, idx is guessed by a truncation of a scaled,
+ *     reassociated v: the terms that do not depend on the previous sample's
+ *     quantizer output are summed first, so only one product and one
+ *     addition separate that output from the next guess; off the chain,
+ *     v in the reference's order is formed and checked against
+ *     [lo[idx], hi[idx]).  A block whose check fails anywhere is
+ *     recomputed from its start state with the reference's formula, so
+ *     every state equals the reference's.
+ * The state update itself (loop filter, y subtraction, clamp) is the
+ * reference's operation order; build with -ffp-contract=off.
+ */
+#include <emmintrin.h>
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+#include "dsdhip_internal.h"
+
+namespace {
+
+inline int x86_trunc(double w) { return _mm_cvttsd_si32(_mm_set_sd(w)); }
+
+/* the reference's quantizer index (src/trellis.c:1162-1168) */
+inline int ref_idx(double v, double a0, double step, int levels)
+{
+    const double yi = v / a0;
+    int idx = x86_trunc((yi + 1.0) / step + 0.5);
+    if (idx < 0) idx = 0;
+    if (idx > levels - 1) idx = levels - 1;
+    return idx;
+}
+
+inline uint64_t dkey(double d)
+{
+    uint64_t u;
+    memcpy(&u, &d, 8);
+    return (u >> 63) ? ~u : (u | (1ull << 63));
+}
+
+inline double dunkey(uint64_t k)
+{
+    const uint64_t u = (k >> 63) ? (k & ~(1ull << 63)) : ~k;
+    double d;
+    memcpy(&d, &u, 8);
+    return d;
+}
+
+struct Quant {
+    int ok;              /* thresholds valid: the fast form may be used */
+    int levels;
+    double a0, step;
+    double lo[32], hi[32], y[32];
+    double over;         /* v >= over: conversion overflow, idx 0 */
+    double c1, c2;       /* idx ~ trunc(v * c1 + c2) */
+};
+
+void quant_init(Quant &q, double a0, int levels)
+{
+    memset(&q, 0, sizeof(q));
+    q.levels = levels;
+    q.a0 = a0;
+    q.step = 2.0 / (double)(levels - 1);
+    for (int k = 0; k < levels && k < 32; k++) q.y[k] = -1.0 + (double)k * q.step;
+    if (!(a0 > 0.0) || levels < 3 || levels > 32 || !isfinite(a0)) return;
+    /* overflow point: first v above a0 * 2 (idx = levels - 1) where idx is 0 */
+    uint64_t l = dkey(2.0 * a0), h = dkey(INFINITY);
+    if (ref_idx(dunkey(l), a0, q.step, levels) != levels - 1) return;
+    while (h - l > 1) {
+        const uint64_t m = l + (h - l) / 2;
+        if (ref_idx(dunkey(m), a0, q.step, levels) == 0) h = m;
+        else l = m;
+    }
+    q.over = dunkey(h);
+    const uint64_t top = h - 1;   /* last v before the overflow */
+    double t[33];
+    t[0] = -INFINITY;
+    for (int k = 1; k < levels; k++) {
+        uint64_t lo = dkey(-INFINITY), hi = top;
+        while (hi - lo > 1) {
+            const uint64_t m = lo + (hi - lo) / 2;
+            if (ref_idx(dunkey(m), a0, q.step, levels) >= k) hi = m;
+            else lo = m;
+        }
+        t[k] = dunkey(hi);
+        /* a step function: exactly k at the point, k-1 just below */
+        if (ref_idx(t[k], a0, q.step, levels) != k || ref_idx(dunkey(hi - 1), a0, q.step, levels) != k - 1)
+            return;
+    }
+    t[levels] = q.over;
+    for (int k = 0; k < levels; k++) {
+        q.lo[k] = t[k];
+        q.hi[k] = t[k + 1];
+    }
+    q.c1 = 1.0 / (a0 * q.step);
+    q.c2 = 1.0 / q.step + 0.5;
+    q.ok = 1;
+}
+
+/* One sample in the reference's form (src/trellis.c:1140-1186). */
+template <int P>
+inline void ref_step(const double *a, const double *g, double *s, double x, const Quant &q, double lim)
+{
+    double ns[P];
+    ns[0] = s[0] - g[0] * s[1] + x;
+    double v = x + a[0] * ns[0];
+    for (int i = 1; i < P - 1; i++) {
+        ns[i] = s[i] + s[i - 1] - g[i] * s[i + 1];
+        v = v + a[i] * ns[i];
+    }
+    ns[P - 1] = s[P - 1] + s[P - 2];
+    v = v + a[P - 1] * ns[P - 1];
+    double y;
+    if (q.levels <= 2) y = (v * a[0] > 0.0) ? 1.0 : -1.0;
+    else y = -1.0 + (double)ref_idx(v, a[0], q.step, q.levels) * q.step;
+    ns[0] = ns[0] - y;
+    if (lim > 0.0)
+        for (int i = 0; i < P; i++) {
+            if (ns[i] > lim) ns[i] = lim;
+            else if (ns[i] < -lim) ns[i] = -lim;
+        }
+    for (int i = 0; i < P; i++) s[i] = ns[i];
+}
+
+/* n samples with the fast index; returns 0 if every index was confirmed
+ * against the reference-order sum (s then holds the exact state), 1 if not
+ * (s is then unspecified). */
+template <int P, bool LIM>
+int fast_block(const double *a, const double *g, double *sv, const double *x, size_t n, const Quant &q,
+               double lim)
+{
+    double s[P];
+    for (int i = 0; i < P; i++) s[i] = sv[i];
+    const int top = q.levels - 1;
+    const double c1 = q.c1, c2 = q.c2, over = q.over;
+    const double e0 = a[0] * c1, e1 = a[1] * c1, es = e0 + e1;
+    int bad = 0;
+    for (size_t t = 0; t < n; t++) {
+        const double xv = x[t];
+        double ns[P], p[P];
+        ns[0] = s[0] - g[0] * s[1] + xv;
+        for (int i = 1; i < P - 1; i++) ns[i] = s[i] + s[i - 1] - g[i] * s[i + 1];
+        ns[P - 1] = s[P - 1] + s[P - 2];
+        for (int i = 0; i < P; i++) p[i] = a[i] * ns[i];
+        /* reference-order sum (checked, off the chain) */
+        double ve = xv + p[0];
+        for (int i = 1; i < P; i++) ve = ve + p[i];
+        /* scaled, reassociated sum: ns0 and ns1 are s0 plus terms known
+         * before the previous output, so only e * s0 sits on the chain */
+        double r = xv + p[2];
+        for (int i = 3; i < P; i++) r = r + p[i];
+        const double base = ((r * c1 + c2) + e0 * (xv - g[0] * s[1])) + e1 * (s[1] - g[1] * s[2]);
+        const double w = base + es * s[0];
+        int k = x86_trunc(w);
+        k = k < 0 ? 0 : (k > top ? top : k);
+        const bool in = (ve >= q.lo[k]) & (ve < q.hi[k]);
+        const bool zero_ok = (k == 0) & ((ve >= over) | (ve != ve));
+        bad |= !(in | zero_ok);
+        ns[0] = ns[0] - q.y[k];
+        if (LIM)
+            for (int i = 0; i < P; i++) {
+                if (ns[i] > lim) ns[i] = lim;
+                else if (ns[i] < -lim) ns[i] = -lim;
+            }
+        for (int i = 0; i < P; i++) s[i] = ns[i];
+    }
+    for (int i = 0; i < P; i++) sv[i] = s[i];
+    return bad;
+}
+
+template <int P>
+void advance(const double *a, const double *g, double *s, const double *x, size_t n, const Quant &q,
+             double lim, int64_t *redo)
+{
+    const size_t BLK = 1024;
+    for (size_t off = 0; off < n; off += BLK) {
+        const size_t m = (n - off < BLK) ? n - off : BLK;
+        if (q.ok) {
+            double keep[P];
+            for (int i = 0; i < P; i++) keep[i] = s[i];
+            const int bad = (lim > 0.0) ? fast_block<P, true>(a, g, s, x + off, m, q, lim)
+                                        : fast_block<P, false>(a, g, s, x + off, m, q, lim);
+            if (!bad) continue;
+            for (int i = 0; i < P; i++) s[i] = keep[i];
+            if (redo) (*redo)++;
+        }
+        for (size_t t = 0; t < m; t++) ref_step<P>(a, g, s, x[off + t], q, lim);
+    }
+}
+
+} /* namespace */
+
+struct dsdhip_est {
+    Quant q;
+    double a[TR_MAXP], g[TR_MAXP];
+    int P;
+    double lim;
+    double s[10];
+    int64_t redo;
+};
+
+extern "C" {
+
+/* Estimator over a stream fed block by block (state carried between calls). */
+dsdhip_est_t *dsdhip_est_create(const double *a, const double *g, int order, double state_limit,
+                                int levels, const double *init10)
+{
+    if (!a || !g || order < 4 || order > TR_MAXP || levels < 2 || levels > 32) return NULL;
+    dsdhip_est_t *e = new dsdhip_est_t();
+    memset(e->a, 0, sizeof(e->a));
+    memset(e->g, 0, sizeof(e->g));
+    memcpy(e->a, a, sizeof(double) * (size_t)order);
+    memcpy(e->g, g, sizeof(double) * (size_t)order);
+    e->P = order;
+    e->lim = state_limit;
+    quant_init(e->q, a[0], levels);
+    for (int i = 0; i < 10; i++) e->s[i] = (init10 && i < order) ? init10[i] : 0.0;
+    e->redo = 0;
+    return e;
+}
+
+void dsdhip_est_destroy(dsdhip_est_t *e) { delete e; }
+
+void dsdhip_est_set_state(dsdhip_est_t *e, const double *init10)
+{
+    for (int i = 0; i < 10; i++) e->s[i] = (i < e->P) ? init10[i] : 0.0;
+}
+
+void dsdhip_est_get_state(const dsdhip_est_t *e, double *out10)
+{
+    for (int i = 0; i < 10; i++) out10[i] = (i < e->P) ? e->s[i] : 0.0;
+}
+
+void dsdhip_est_feed(dsdhip_est_t *e, const double *x, size_t n)
+{
+    switch (e->P) {
+    case 4: advance<4>(e->a, e->g, e->s, x, n, e->q, e->lim, &e->redo); break;
+    case 5: advance<5>(e->a, e->g, e->s, x, n, e->q, e->lim, &e->redo); break;
+    case 6: advance<6>(e->a, e->g, e->s, x, n, e->q, e->lim, &e->redo); break;
+    case 7: advance<7>(e->a, e->g, e->s, x, n, e->q, e->lim, &e->redo); break;
+    default: advance<8>(e->a, e->g, e->s, x, n, e->q, e->lim, &e->redo); break;
+    }
+}
+
+int64_t dsdhip_est_redo_blocks(const dsdhip_est_t *e) { return e->redo; }
+int dsdhip_est_fast(const dsdhip_est_t *e) { return e->q.ok; }
+
+/* The estimation block of src/threadpool.c:345-366 on host memory: from
+ * init over x[0..count), boundaries bnd[0..nb) (offsets into x, the state
+ * being advanced to a boundary only when it lies after the current position
+ * and within count); out[b*10..] = the state at boundary b.  Pure host
+ * code.  index_bias shifts the fast index guess (validation: the results
+ * must not depend on it); redo_blocks receives the blocks recomputed in the
+ * reference form. */
+int dsdhip_estimate_boundaries_ex(const double *a, const double *g, int order, const double *init10,
+                                  const double *x, size_t count, const int64_t *bnd, int nb,
+                                  double state_limit, int levels, double *out10, double index_bias,
+                                  int64_t *redo_blocks)
+{
+    if (!a || !g || !init10 || (!x && count) || (nb > 0 && (!bnd || !out10)) || order < 4 ||
+        order > TR_MAXP || levels < 2 || levels > 32)
+        return -1;
+    dsdhip_est_t *e = dsdhip_est_create(a, g, order, state_limit, levels, init10);
+    if (!e) return -1;
+    e->q.c2 += index_bias;
+    size_t pos = 0;
+    for (int b = 0; b < nb; b++) {
+        const int64_t bd = bnd[b];
+        if (bd > (int64_t)pos && (size_t)bd <= count) {
+            dsdhip_est_feed(e, x + pos, (size_t)bd - pos);
+            pos = (size_t)bd;
+        }
+        dsdhip_est_get_state(e, out10 + (size_t)b * 10);
+    }
+    if (redo_blocks) *redo_blocks = e->redo;
+    dsdhip_est_destroy(e);
+    return 0;
+}
+
+int dsdhip_estimate_boundaries(const double *a, const double *g, int order, const double *init10,
+                               const double *x, size_t count, const int64_t *bnd, int nb,
+                               double state_limit, int levels, double *out10)
+{
+    return dsdhip_estimate_boundaries_ex(a, g, order, init10, x, count, bnd, nb, state_limit, levels, out10,
+                                         0.0, NULL);
+}
+
+/* The quantizer index the fast form's exact interval table assigns to v
+ * (levels 3..32, a0 > 0), or -1 when the table is not used for (a0,
+ * levels).  For checking the table against the reference formula. */
+int dsdhip_est_table_index(double a0, int levels, double v)
+{
+    Quant q;
+    quant_init(q, a0, levels);
+    if (!q.ok) return -1;
+    if (v >= q.over || v != v) return 0;
+    for (int k = levels - 1; k >= 0; k--)
+        if (v >= q.lo[k]) return k;
+    return 0;
+}
+
+} /* extern "C" */

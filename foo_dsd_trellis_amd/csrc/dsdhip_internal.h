@@ -21,6 +21,14 @@ int dsdhip_trellis_detach(dsdhip_trellis_t *t, int i);
 int dsdhip_trellis_copy_detached(dsdhip_trellis_t *t, int dst, int src);
 int dsdhip_trellis_params(const dsdhip_trellis_t *t, tr_params *out);
 void *dsdhip_trellis_context_ptr(dsdhip_trellis_t *t, int i);
+/* pipelined job table (dsdhip_trellis.cpp) */
+int dsdhip_trellis_table_begin(dsdhip_trellis_t *t, int njobs);
+int dsdhip_trellis_table_fill(dsdhip_trellis_t *t, int j, int ctx, int copy_src, const double *d_in,
+                              int64_t n_in, int64_t keep_from, uint64_t *d_out, const double *seed10);
+int dsdhip_trellis_table_launch(dsdhip_trellis_t *t, int first, int n, void *stream);
+int dsdhip_trellis_table_end(dsdhip_trellis_t *t, void *stream, int64_t *kept, int64_t *slow_steps,
+                             int *err_out);
+int dsdhip_trellis_timing_add(dsdhip_trellis_t *t, double ms, int64_t launches);
 /* 255-tap Kaiser low-pass of the stitch quality (src/dsp_plugin.c:2571-2604). */
 void dsdhip_make_qfir(double *q);
 /* Device scratch of the FIR chain for n input samples per channel, so later
@@ -28,6 +36,18 @@ void dsdhip_make_qfir(double *q);
 int dsdhip_fir_reserve(dsdhip_fir_t *f, size_t n);
 /* f64 -> f32 copy on the device (round to nearest). */
 int dsdhip_f64_to_f32_dev(const double *d_x, float *d_y, size_t n, void *stream);
+
+/* Host estimator of sdm_estimate_state_multibit fed block by block
+ * (dsdhip_estimate.cpp); one object per channel, one thread at a time. */
+typedef struct dsdhip_est dsdhip_est_t;
+dsdhip_est_t *dsdhip_est_create(const double *a, const double *g, int order, double state_limit,
+                                int levels, const double *init10);
+void dsdhip_est_destroy(dsdhip_est_t *e);
+void dsdhip_est_set_state(dsdhip_est_t *e, const double *init10);
+void dsdhip_est_get_state(const dsdhip_est_t *e, double *out10);
+void dsdhip_est_feed(dsdhip_est_t *e, const double *x, size_t n);
+int64_t dsdhip_est_redo_blocks(const dsdhip_est_t *e);
+int dsdhip_est_fast(const dsdhip_est_t *e);
 
 #ifdef __cplusplus
 }

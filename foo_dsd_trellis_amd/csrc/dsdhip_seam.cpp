@@ -32,7 +32,9 @@ struct gpu_hip_context {
     dsdhip_das_t *das;
     int das_nch;
     size_t das_seg_len;
-    int das_seg0_first, das_reposition;
+    int das_seg0_first, das_reposition;   /* das_seg0_first < 0: by output rate */
+    unsigned das_rate;                    /* output rate for that rule (0: unknown) */
+    size_t das_count;                     /* samples per channel of the last DAS call */
     int fma_o8;
     /* uniform-taps FIR setup (gpu_hip_fir_setup) */
     double taps[127];
@@ -59,7 +61,7 @@ void *gpu_hip_create(void)
     gpu_hip_context *c = (gpu_hip_context *)calloc(1, sizeof(gpu_hip_context));
     c->backend = GPU_BACKEND_HIP;
     c->das_seg_len = 65536;
-    c->das_seg0_first = 0;
+    c->das_seg0_first = -1;
     c->das_reposition = 1;
     return c;
 }
@@ -225,6 +227,11 @@ int gpu_hip_fir_setup_rates(void *ctx, unsigned fs_in, unsigned fs_out, double g
     c->fir_ch = 0;
     c->fs_in = fs_in;
     c->fs_out = fs_out;
+    if (c->das_rate != fs_out) {
+        c->das_rate = fs_out;
+        dsdhip_das_destroy(c->das);
+        c->das = NULL;
+    }
     c->fir_gain = gain;
     c->ntaps = 0;
     return 0;
@@ -340,10 +347,22 @@ int gpu_hip_das_config(void *ctx, size_t seg_len, int seg0_first, int reposition
     gpu_hip_context *c = (gpu_hip_context *)ctx;
     if (!c) return -1;
     c->das_seg_len = seg_len;
-    c->das_seg0_first = seg0_first;
+    c->das_seg0_first = seg0_first < 0 ? -1 : (seg0_first ? 1 : 0);
     c->das_reposition = reposition;
     dsdhip_das_destroy(c->das);
     c->das = NULL;
+    return 0;
+}
+
+int gpu_hip_das_rate(void *ctx, unsigned fs_out)
+{
+    gpu_hip_context *c = (gpu_hip_context *)ctx;
+    if (!c) return -1;
+    if (c->das_rate != fs_out) {
+        c->das_rate = fs_out;
+        dsdhip_das_destroy(c->das);
+        c->das = NULL;
+    }
     return 0;
 }
 
@@ -357,7 +376,16 @@ int gpu_hip_trellis_fma_o8(void *ctx, int on)
     return 0;
 }
 
-int gpu_hip_trellis_das(void *ctx, const double *in, float *out, size_t count, int num_channels)
+/* seed mode of the reference's DAS (src/dsp_plugin.c:2185): seg0-first
+ * below DSD512, all segments from the persistent state at DSD512 and up */
+static int seed_mode(const gpu_hip_context *c)
+{
+    if (c->das_seg0_first >= 0) return c->das_seg0_first;
+    return (c->das_rate == 0 || c->das_rate < DSDHIP_DSD512_RATE) ? 1 : 0;
+}
+
+int gpu_hip_trellis_das_ex(void *ctx, const double *in, float *out, size_t count, int num_channels,
+                           size_t *out_count)
 {
     gpu_hip_context *c = (gpu_hip_context *)ctx;
     if (!c || !c->have_tr || !in || !out || num_channels < 1) return -1;
@@ -365,14 +393,28 @@ int gpu_hip_trellis_das(void *ctx, const double *in, float *out, size_t count, i
         dsdhip_das_destroy(c->das);
         c->das = dsdhip_das_create_coeffs(c->a, c->g, c->order, c->depth, c->cands, c->lat,
                                           c->state_limit, num_channels, c->das_seg_len,
-                                          c->das_seg_len ? 0 : 2, c->das_seg0_first,
-                                          c->das_reposition);
+                                          c->das_seg_len ? 0 : 2, seed_mode(c), c->das_reposition);
         c->das_nch = num_channels;
         if (!c->das) return -1;
         dsdhip_das_fma_o8(c->das, c->fma_o8);
     }
     size_t n = 0;
-    return dsdhip_das_run(c->das, in, count, out, &n);
+    c->das_count = 0;
+    const int rc = dsdhip_das_run(c->das, in, count, out, &n);
+    if (rc == 0) c->das_count = n;
+    if (out_count) *out_count = rc == 0 ? n : 0;
+    return rc;
+}
+
+int gpu_hip_trellis_das(void *ctx, const double *in, float *out, size_t count, int num_channels)
+{
+    return gpu_hip_trellis_das_ex(ctx, in, out, count, num_channels, NULL);
+}
+
+size_t gpu_hip_trellis_das_count(void *ctx)
+{
+    gpu_hip_context *c = (gpu_hip_context *)ctx;
+    return c ? c->das_count : 0;
 }
 
 } /* extern "C" */

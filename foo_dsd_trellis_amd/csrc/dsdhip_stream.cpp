@@ -100,8 +100,11 @@ dsdhip_stream_t *dsdhip_stream_create_cfg(const dsdhip_stream_cfg_t *cfg)
         s->conv_block = 8 * cfg->P;
         ok = ok && s->cap % (size_t)s->conv_block == 0;
     }
+    /* seeding: the reference's rule by output rate (src/dsp_plugin.c:2185) */
+    const int seg0_first = cfg->seeding == 1 || (cfg->seeding == 0 && cfg->fs_out < DSDHIP_DSD512_RATE);
+    ok = ok && cfg->seeding >= 0 && cfg->seeding <= 2;
     ok = ok && (s->das = dsdhip_das_create(cfg->ntf_index, cfg->depth, cfg->cands, cfg->lat, cfg->state_limit,
-                                            nch, cfg->seg_len, 2, 0, cfg->reposition)) != nullptr;
+                                            nch, cfg->seg_len, 2, seg0_first, cfg->reposition)) != nullptr;
     if (ok && cfg->fma_o8) dsdhip_das_fma_o8(s->das, 1);
     ok = ok && hipMalloc(&s->d_fir, sizeof(double) * s->cap * (size_t)nch) == hipSuccess;
     if (ok) {

@@ -35,6 +35,9 @@ struct dsdhip_trellis {
     double *d_pin;           /* host-buffer entry: staged input / packed output, grow-only */
     uint64_t *d_pout;
     size_t pin_cap, pout_cap;
+    tr_job *table;               /* pipelined job table, pinned host memory (grow-only) */
+    size_t table_cap, table_n;
+    std::vector<int> table_src;  /* copy source of each entry */
 };
 
 static void ctx_fresh(tr_ctx *c)
@@ -74,6 +77,8 @@ dsdhip_trellis_t *dsdhip_trellis_create_coeffs(const double *a, const double *g,
     t->d_pin = NULL;
     t->d_pout = NULL;
     t->pin_cap = t->pout_cap = 0;
+    t->table = NULL;
+    t->table_cap = t->table_n = 0;
     t->stream = 0;
     t->timing = 0;
     t->kernel_ms = 0.0;
@@ -155,6 +160,7 @@ void dsdhip_trellis_destroy(dsdhip_trellis_t *t)
     if (t->d_jobs) hipFree(t->d_jobs);
     if (t->d_pin) hipFree(t->d_pin);
     if (t->d_pout) hipFree(t->d_pout);
+    if (t->table) hipHostFree(t->table);
     delete t;
 }
 
@@ -306,6 +312,122 @@ int dsdhip_trellis_best_state(dsdhip_trellis_t *t, int i, double *out)
  * final state back into ctx[j].  kept[j] receives the kept output count.
  * Returns 0, or -1 on launch failure / error flag (err_out gets the flag).
  */
+/* one job descriptor; -1 on a context index outside the pool */
+static int fill_job(const dsdhip_trellis_t *t, tr_job &jb, int ctx, int src, const double *d_in,
+                    int64_t n_in, int64_t keep_from, uint64_t *d_out, const double *seed10)
+{
+    memset(&jb, 0, sizeof(jb));
+    if (ctx < 0 || ctx >= t->nctx || src >= t->nctx) return -1;
+    jb.in = d_in;
+    jb.out = d_out;
+    jb.n_in = n_in;
+    jb.keep_from = keep_from;
+    jb.ctx_out = ctx;
+    if (src >= 0) {
+        jb.ctx_in = src;
+        jb.copy = 1;
+        jb.snap = src;
+    } else {
+        jb.ctx_in = ctx;
+        jb.copy = 0;
+        jb.snap = t->snap_of[ctx];
+    }
+    if (seed10 && !(seed10[0] != seed10[0])) {   /* NaN in slot 0 = no seed */
+        jb.seed = 1;
+        for (int q = 0; q < TR_MAXP; q++) jb.seed_st[q] = seed10[q];
+    }
+    return 0;
+}
+
+static int ensure_jobs(dsdhip_trellis_t *t, int njobs)
+{
+    if (t->jobs_cap >= njobs) return 0;
+    if (t->d_jobs) hipFree(t->d_jobs);
+    t->d_jobs = NULL;
+    t->jobs_cap = 0;
+    if (hipMalloc(&t->d_jobs, sizeof(tr_job) * (size_t)njobs) != hipSuccess) return -1;
+    t->jobs_cap = njobs;
+    return 0;
+}
+
+/*
+ * Pipelined launches over one job table: entries are filled on the host and
+ * launched in batches as their inputs become known (each batch on the
+ * stream the caller names, so batches overlap), and the results of every
+ * entry are read back once at the end.  The pool's device job table is
+ * grown to njobs first; nothing else of the pool may launch until
+ * dsdhip_trellis_table_end.
+ */
+int dsdhip_trellis_table_begin(dsdhip_trellis_t *t, int njobs)
+{
+    if (!t || njobs < 0) return -1;
+    if (ensure_jobs(t, njobs > 0 ? njobs : 1) != 0) return -1;
+    if (t->table_cap < (size_t)njobs) {
+        if (t->table) hipHostFree(t->table);
+        t->table = NULL;
+        t->table_cap = 0;
+        if (hipHostMalloc((void **)&t->table, sizeof(tr_job) * (size_t)njobs, 0) != hipSuccess) return -1;
+        t->table_cap = (size_t)njobs;
+    }
+    t->table_n = (size_t)njobs;
+    t->table_src.assign((size_t)njobs, -1);
+    return 0;
+}
+
+int dsdhip_trellis_table_fill(dsdhip_trellis_t *t, int j, int ctx, int copy_src, const double *d_in,
+                              int64_t n_in, int64_t keep_from, uint64_t *d_out, const double *seed10)
+{
+    if (!t || j < 0 || (size_t)j >= t->table_n) return -1;
+    t->table_src[(size_t)j] = copy_src;
+    return fill_job(t, t->table[j], ctx, copy_src, d_in, n_in, keep_from, d_out, seed10);
+}
+
+int dsdhip_trellis_table_launch(dsdhip_trellis_t *t, int first, int n, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (!t || first < 0 || n < 0 || (size_t)first + (size_t)n > t->table_n) return -1;
+    if (n == 0) return 0;
+    if (hipMemcpyAsync(t->d_jobs + first, t->table + first, sizeof(tr_job) * (size_t)n,
+                       hipMemcpyHostToDevice, s) != hipSuccess)
+        return -1;
+    return tr_launch_jobs(&t->prm, t->d_jobs + first, n, t->d_ctx, s);
+}
+
+/* After every batch has been joined into stream s: results of all entries. */
+int dsdhip_trellis_table_end(dsdhip_trellis_t *t, void *stream, int64_t *kept, int64_t *slow_steps,
+                             int *err_out)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (!t) return -1;
+    const size_t n = t->table_n;
+    if (n && hipMemcpyAsync(t->table, t->d_jobs, sizeof(tr_job) * n, hipMemcpyDeviceToHost, s) != hipSuccess)
+        return -1;
+    if (hipStreamSynchronize(s) != hipSuccess) return -1;
+    int err = 0;
+    for (size_t j = 0; j < n; j++) {
+        if (kept) kept[j] = t->table[j].kept;
+        if (slow_steps) slow_steps[j] = t->table[j].slow_steps;
+        if (t->table[j].err && !err) err = t->table[j].err;
+        if (t->table_src[j] >= 0) t->snap_of[t->table[j].ctx_out] = t->table_src[j];
+    }
+    if (t->timing)
+        for (size_t j = 0; j < n; j++) t->samples += t->table[j].n_in;
+    if (err_out) *err_out = err;
+    return err ? -1 : 0;
+}
+
+/* Adds a measured span to the launch timing (pipelined runs time their
+ * trellis work as one span). */
+int dsdhip_trellis_timing_add(dsdhip_trellis_t *t, double ms, int64_t launches)
+{
+    if (!t) return -1;
+    if (t->timing) {
+        t->kernel_ms += ms;
+        t->kernel_launches += launches;
+    }
+    return 0;
+}
+
 int dsdhip_trellis_run_jobs(dsdhip_trellis_t *t, int njobs, const int *ctx, const int *copy_src,
                             const double *const *d_in, const int64_t *n_in,
                             const int64_t *keep_from, uint64_t *const *d_out,
@@ -316,37 +438,12 @@ int dsdhip_trellis_run_jobs(dsdhip_trellis_t *t, int njobs, const int *ctx, cons
     if (njobs == 0) return 0;
     std::vector<tr_job> jobs((size_t)njobs);
     for (int j = 0; j < njobs; j++) {
-        tr_job &jb = jobs[(size_t)j];
-        memset(&jb, 0, sizeof(jb));
-        if (ctx[j] < 0 || ctx[j] >= t->nctx) return -1;
-        int src = (copy_src && copy_src[j] >= 0) ? copy_src[j] : -1;
-        jb.in = d_in[j];
-        jb.out = d_out[j];
-        jb.n_in = n_in[j];
-        jb.keep_from = keep_from ? keep_from[j] : 0;
-        jb.ctx_out = ctx[j];
-        if (src >= 0) {
-            if (src >= t->nctx) return -1;
-            jb.ctx_in = src;
-            jb.copy = 1;
-            jb.snap = src;
-        } else {
-            jb.ctx_in = ctx[j];
-            jb.copy = 0;
-            jb.snap = t->snap_of[ctx[j]];
-        }
-        if (seed && !(seed[j * 10] != seed[j * 10])) {   /* NaN in slot 0 = no seed */
-            jb.seed = 1;
-            for (int q = 0; q < TR_MAXP; q++) jb.seed_st[q] = seed[j * 10 + q];
-        }
+        const int src = (copy_src && copy_src[j] >= 0) ? copy_src[j] : -1;
+        if (fill_job(t, jobs[(size_t)j], ctx[j], src, d_in[j], n_in[j], keep_from ? keep_from[j] : 0,
+                     d_out[j], seed ? seed + (size_t)j * 10 : NULL) != 0)
+            return -1;
     }
-    if (t->jobs_cap < njobs) {
-        if (t->d_jobs) hipFree(t->d_jobs);
-        t->d_jobs = NULL;
-        t->jobs_cap = 0;
-        if (hipMalloc(&t->d_jobs, sizeof(tr_job) * (size_t)njobs) != hipSuccess) return -1;
-        t->jobs_cap = njobs;
-    }
+    if (ensure_jobs(t, njobs) != 0) return -1;
     if (hipMemcpyAsync(t->d_jobs, jobs.data(), sizeof(tr_job) * (size_t)njobs,
                        hipMemcpyHostToDevice, t->stream) != hipSuccess)
         return -1;

@@ -8,6 +8,55 @@ from ._lib import DsdHipError, check, lib
 
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
+_i64p = C.POINTER(C.c_int64)
+
+
+def _coeffs(ntf_index):
+    row = lib().dsdhip_ntf_get(ntf_index)
+    if not row:
+        raise DsdHipError("no NTF filter %d" % ntf_index)
+    r = row.contents
+    return np.array(r.a[:], np.float64), np.array(r.g[:], np.float64), r.order
+
+
+def estimate_boundaries(ntf_index, init, x, bnd, state_limit=0.0, levels=16, index_bias=0.0):
+    """The estimation block of src/threadpool.c:345-366 on the host: the
+    16-level greedy recurrence (src/trellis.c:1118-1187) from init over x,
+    the state at each boundary offset in bnd -> ((nb, 10) states, blocks
+    recomputed in the reference form).  index_bias is a validation switch."""
+    a, g, order = _coeffs(ntf_index)
+    init = np.zeros(10) if init is None else np.asarray(init, np.float64)
+    init = np.concatenate([init, np.zeros(10 - init.size)]) if init.size < 10 else init[:10].copy()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    bnd = np.ascontiguousarray(bnd, dtype=np.int64)
+    out = np.zeros((max(1, bnd.size), 10))
+    redo = C.c_int64(0)
+    check(lib().dsdhip_estimate_boundaries_ex(a.ctypes.data_as(_dp), g.ctypes.data_as(_dp), order,
+                                              init.ctypes.data_as(_dp), x.ctypes.data_as(_dp), x.size,
+                                              bnd.ctypes.data_as(_i64p), bnd.size, state_limit, levels,
+                                              out.ctypes.data_as(_dp), index_bias, C.byref(redo)),
+          "dsdhip_estimate_boundaries")
+    return out[:bnd.size], redo.value
+
+
+def estimate_boundaries_dev(ntf_index, init, d_x, stride, nch, count, bnd, state_limit=0.0, levels=16):
+    """The same on one device lane per channel: d_x a device pointer to nch
+    rows of `stride` doubles; init (nch, 10) -> (nch, nb, 10)."""
+    a, g, order = _coeffs(ntf_index)
+    init = np.ascontiguousarray(np.broadcast_to(np.asarray(init, np.float64), (nch, 10)))
+    bnd = np.ascontiguousarray(bnd, dtype=np.int64)
+    out = np.zeros((nch, bnd.size, 10))
+    check(lib().dsdhip_estimate_boundaries_dev(a.ctypes.data_as(_dp), g.ctypes.data_as(_dp), order,
+                                               init.ctypes.data_as(_dp), d_x, stride, nch, count,
+                                               bnd.ctypes.data_as(_i64p), bnd.size, state_limit, levels,
+                                               out.ctypes.data_as(_dp)), "dsdhip_estimate_boundaries_dev")
+    return out
+
+
+def table_index(a0, levels, v):
+    """Quantizer index from the host estimator's exact interval table (-1:
+    table not used for these parameters)."""
+    return lib().dsdhip_est_table_index(a0, levels, v)
 
 
 class Das:
@@ -79,6 +128,22 @@ class Das:
         check(lib().dsdhip_das_timing_read(self.h, C.byref(ms), C.byref(nl), C.byref(ns),
                                            1 if clear else 0), "timing_read")
         return ms.value, nl.value, ns.value
+
+    def estimator_device(self, on=True):
+        """seg0-first: run the seeding estimate on one device lane per channel
+        instead of host threads (timing only; same bits)."""
+        check(lib().dsdhip_das_estimator_device(self.h, 1 if on else 0), "estimator_device")
+
+    def last_estimate(self):
+        """(longest estimator chain ms, samples estimated, redo blocks, batches)
+        of the last seg0-first run."""
+        ms = C.c_double(0)
+        ns = C.c_int64(0)
+        rd = C.c_int64(0)
+        nb = C.c_int(0)
+        check(lib().dsdhip_das_last_estimate(self.h, C.byref(ms), C.byref(ns), C.byref(rd), C.byref(nb)),
+              "last_estimate")
+        return ms.value, ns.value, rd.value, nb.value
 
     def last_info(self, max_pos=0):
         ns = C.c_int(0)

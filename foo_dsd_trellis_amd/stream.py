@@ -44,7 +44,11 @@ class StreamCfg(C.Structure):
                 ("soft_clip", C.c_int), ("dsd_boxcar", C.c_int), ("dsd_lowpass", C.c_int), ("ir", _dp),
                 ("ir_len", C.c_int64), ("ir_stride", C.c_int64), ("P", C.c_int), ("ntf_index", C.c_int),
                 ("depth", C.c_int), ("cands", C.c_int), ("lat", C.c_int), ("state_limit", C.c_double),
-                ("seg_len", C.c_size_t), ("reposition", C.c_int), ("fma_o8", C.c_int)]
+                ("seg_len", C.c_size_t), ("reposition", C.c_int), ("fma_o8", C.c_int), ("seeding", C.c_int)]
+
+
+# DAS seeding of the chain: the reference's rule by output rate, or forced
+SEED_BY_RATE, SEED_SEG0_FIRST, SEED_ALL_PARALLEL = 0, 1, 2
 
 
 class Stream:
@@ -54,8 +58,10 @@ class Stream:
     def __init__(self, fs_in, fs_out, nch, chunk_frames, ir=None, P=8192, ntf_index=None, K=32, L=64,
                  depth=8, gain=fir.PCM_FIR_GAIN, state_limit=0.0, seg_len=65536, reposition=True,
                  input=IN_PCM, volume=1.0, pcm_boxcar=0, soft_clip=False, dsd_boxcar=0, dsd_lowpass=False,
-                 fma_o8=False):
-        """gain: the engine's fir_gain; volume: cfg->gain (set_gain ramps it)."""
+                 fma_o8=False, seeding=SEED_BY_RATE):
+        """gain: the engine's fir_gain; volume: cfg->gain (set_gain ramps it);
+        seeding: SEED_BY_RATE (seg0-first below DSD512, as the reference),
+        SEED_SEG0_FIRST or SEED_ALL_PARALLEL."""
         self.lib = _lib.lib()
         _sigs(self.lib)
         if ntf_index is None:
@@ -72,6 +78,8 @@ class Stream:
             c.ir, c.ir_len, c.ir_stride = self._ir.ctypes.data_as(_dp), self._ir.shape[1], self._ir.shape[1]
         c.P, c.ntf_index, c.depth, c.cands, c.lat = P, ntf_index, depth, K, L
         c.state_limit, c.seg_len, c.reposition, c.fma_o8 = state_limit, seg_len, int(reposition), int(fma_o8)
+        c.seeding = seeding
+        self.seg0_first = seeding == SEED_SEG0_FIRST or (seeding == SEED_BY_RATE and fs_out < 22579200)
         self.h = self.lib.dsdhip_stream_create_cfg(C.byref(c))
         if not self.h:
             raise _lib.DsdHipError("dsdhip_stream_create_cfg failed")

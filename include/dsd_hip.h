@@ -82,9 +82,19 @@ int gpu_hip_trellis_setup(void *ctx, int num_cands, int order, int trellis_lat,
  * sdm_process_block (src/trellis.c:1013-1040) would produce. */
 int gpu_hip_trellis(void *ctx, const double *in, float *out, size_t count);
 /* replaces gpu_cuda_trellis_das (include/gpu_compute.h:232-233): DAS over
- * num_channels x count samples (channel-major), all segments in one launch,
- * same semantics as the reference's CPU DAS (src/dsp_plugin.c:1314-2761). */
+ * num_channels x count samples (channel-major), same semantics as the
+ * reference's CPU DAS (src/dsp_plugin.c:1314-2761).  Each output row holds
+ * the produced samples (+-1.0) followed by 0.0f up to count.  With the
+ * reference's seeding for the rate (seg0-first below DSD512) the produced
+ * count is count, or count - trellis_lat on a fresh context, as the
+ * plugin's output check expects (src/dsp_plugin.c:2733-2747). */
 int gpu_hip_trellis_das(void *ctx, const double *in, float *out, size_t count, int num_channels);
+/* gpu_hip_trellis_das that also returns the produced samples per channel
+ * (extension of the seam; 0 on failure). */
+int gpu_hip_trellis_das_ex(void *ctx, const double *in, float *out, size_t count, int num_channels,
+                           size_t *out_count);
+/* Produced samples per channel of the last gpu_hip_trellis_das(_ex) call. */
+size_t gpu_hip_trellis_das_count(void *ctx);
 /* Bit parity with the shipped Windows build: order-8 filters computed by
  * the reference's AVX2 4-candidate FMA path (src/trellis.c:348-533, chosen
  * at :904-927 for the first 4*floor(n/4) candidates of every step) instead
@@ -108,8 +118,15 @@ int gpu_hip_fir_lowpass(void *ctx, const float *in, float *out, size_t count, fl
 int gpu_hip_boxcar_smooth(void *ctx, const float *in, float *out, size_t count, int taps, float gain);
 /* DAS plan parameters for gpu_hip_trellis_das (segment length in samples,
  * 0 = reference auto count; seed mode 1 = seg0-first, 0 = all segments
- * seeded from the persistent state; reposition quiet boundaries). */
+ * seeded from the persistent state, DSDHIP_SEED_BY_RATE (the default) =
+ * the reference's rule, seg0-first below DSD512 (src/dsp_plugin.c:2185);
+ * reposition quiet boundaries). */
+#define DSDHIP_SEED_BY_RATE (-1)
+#define DSDHIP_DSD512_RATE 22579200u
 int gpu_hip_das_config(void *ctx, size_t seg_len, int seg0_first, int reposition);
+/* Output rate for the seed rule (also taken from gpu_hip_fir_setup_rates);
+ * an unknown rate (0) seeds seg0-first. */
+int gpu_hip_das_rate(void *ctx, unsigned fs_out);
 
 /* ------------------------------------------------------------------------
  * 2. NTF tables — include/ntf.h:23-36
@@ -163,10 +180,37 @@ int dsdhip_trellis_drain(dsdhip_trellis_t *t, int ctx, float *out, size_t max_ou
 int dsdhip_trellis_state_distance(dsdhip_trellis_t *t, int i, int j, double *dist);
 /* replaces sdm_estimate_state_multibit (include/trellis.h:114-118); with
  * num_levels = 16 it is sdm_estimate_state (:108-110).  Host buffers; the
- * greedy recurrence runs on one device lane. */
+ * greedy recurrence (one dependent chain, latency-bound) runs on the
+ * calling thread, bit-exact with the reference. */
 int dsdhip_estimate_state(const double *a, const double *g, int order, const double *init_state,
                           const double *in, size_t count, double state_limit, int num_levels,
                           double *out_state);
+/* replaces the estimation block of src/threadpool.c:345-366
+ * (BLOCK_MODE_ESTIMATE): from init10 over x[0..count), the state at each
+ * boundary offset bnd[0..nb) into out10[nb][10] (a boundary advances the
+ * state only when it lies past the current position and within count).
+ * Host memory, calling thread, no device use. */
+int dsdhip_estimate_boundaries(const double *a, const double *g, int order, const double *init10,
+                               const double *x, size_t count, const int64_t *bnd, int nb,
+                               double state_limit, int levels, double *out10);
+/* Validation form: index_bias shifts the fast form's index guess (results
+ * must not change: wrong guesses send their block to the reference form);
+ * redo_blocks receives the blocks so recomputed.  No reference counterpart. */
+int dsdhip_estimate_boundaries_ex(const double *a, const double *g, int order, const double *init10,
+                                  const double *x, size_t count, const int64_t *bnd, int nb,
+                                  double state_limit, int levels, double *out10, double index_bias,
+                                  int64_t *redo_blocks);
+/* Index that the fast form's exact interval table gives v (-1: the table is
+ * not used for this a0 / level count).  No reference counterpart. */
+int dsdhip_est_table_index(double a0, int levels, double v);
+/* The same recurrence on one device lane per channel (the DAS kernel
+ * das_estimate): x is device memory, nch rows of stride doubles starting at
+ * `start`; init10/out10 host arrays [nch][10] / [nch][nb][10].  For timing
+ * against the host form; DAS runs the host form. */
+int dsdhip_estimate_boundaries_dev(const double *a, const double *g, int order, const double *init10,
+                                   const double *d_x, int64_t stride, int nch, size_t count,
+                                   const int64_t *bnd, int nb, double state_limit, int levels,
+                                   double *out10);
 /* Batched form, device buffers (see dsdhip_trellis.cpp for field meaning):
  * one launch, one wave per job; outputs are packed bits (1 = +1.0). */
 int dsdhip_trellis_run_jobs(dsdhip_trellis_t *t, int njobs, const int *ctx, const int *copy_src,
@@ -247,7 +291,8 @@ int dsdhip_das_plan(const dsdhip_das_t *d, size_t F, size_t *start, size_t *size
  * of out_words packed-bit words (bit = 1 for +1.0). */
 int dsdhip_das_run_dev(dsdhip_das_t *d, const double *d_in, int64_t in_stride, size_t F,
                        unsigned long long *d_out, int64_t out_words, size_t *out_count);
-/* Host buffers: in nch x F doubles, out nch x F floats (+-1.0). */
+/* Host buffers: in nch x F doubles, out nch x F floats: *out_count
+ * samples of +-1.0 per row, then 0.0f up to F. */
 int dsdhip_das_run(dsdhip_das_t *d, const double *in, size_t F, float *out, size_t *out_count);
 /* Fresh persistent contexts (sdm_context_reset, include/trellis.h:86). */
 int dsdhip_das_reset(dsdhip_das_t *d);
@@ -268,24 +313,40 @@ int dsdhip_das_run_tracks_dev(dsdhip_das_t *d, const double *const *d_in, const 
 /* Segment count, literal-prune steps and stitch positions of the last run. */
 int dsdhip_das_last_info(const dsdhip_das_t *d, int *nseg, int64_t *slow_steps, int *stitch_pos,
                          int max_pos);
+/* seg0-first seeding runs its estimate (src/threadpool.c:345-366) on host
+ * threads, one chain per channel, and launches each segment as soon as the
+ * estimate has passed its start; on = 1 runs the estimate on one device lane
+ * per channel before one launch instead (same bits; for timing).  No
+ * reference counterpart. */
+int dsdhip_das_estimator_device(dsdhip_das_t *d, int on);
+/* Estimator figures of the last seg0-first run: longest chain's busy time,
+ * samples estimated over all channels, blocks the fast form handed back to
+ * the reference form, pipelined trellis launches. */
+int dsdhip_das_last_estimate(const dsdhip_das_t *d, double *busy_ms, int64_t *samples,
+                             int64_t *redo_blocks, int *batches);
 
 /* ======================================================================
- * 6. Batched DAS: many tracks of uneven length in one persistent launch.
- *    No counterpart in the reference (its DAS runs one stream per plugin
- *    instance, src/dsp_plugin.c:1941-2761); a track is nch channels of
- *    float32 PCM in device memory.  Every chunk step (chunk_frames PCM
- *    frames of every unfinished track: FIR chain, segment plan with
- *    reposition, one persistent trellis launch over all segments of all
- *    channels and tracks, stitch, assembly, copy-back) runs on the device
- *    and is captured once in a hipGraph that is replayed per chunk.  Per
- *    track and chunk the result equals the single-stream DAS with
- *    all-parallel seeding and the same segment length.
+ * 6. Batched DAS: many tracks of uneven length, one trellis launch per
+ *    chunk step.  No counterpart in the reference (its DAS runs one stream
+ *    per plugin instance, src/dsp_plugin.c:1941-2761); a track is nch
+ *    channels of float32 PCM in device memory.  Every chunk step
+ *    (chunk_frames PCM frames of every unfinished track) runs one FIR chain
+ *    over all streams, the segment plans with reposition, one trellis launch
+ *    over all segments of all channels and tracks (with seg0-first seeding:
+ *    seg0 first, then the pipelined rest), one stitch launch and the
+ *    assembly; the host orchestrates the step.  Per track and chunk the
+ *    result equals the single-stream DAS with the reference's seeding for
+ *    fs_out (seg0-first below DSD512, src/dsp_plugin.c:2185) and the same
+ *    segment length.  A device-side form of the step, made to be captured
+ *    once in a hipGraph and replayed per chunk, is built but switched off:
+ *    plans asking for it (use_graph = 1) run the host step and report
+ *    use_graph = 0.
  * ====================================================================== */
 typedef struct {
     int seg_len;            /* segment length; 0: chosen by the autotuner */
-    int waves_per_group;    /* persistent launch: waves per workgroup (1, 2, 4); 0: tuned */
-    int waves_per_cu;       /* persistent launch: resident waves per CU; 0: tuned */
-    int use_graph;          /* 1: capture one chunk step and replay it (default) */
+    int waves_per_group;    /* device step's persistent launch: waves per workgroup (1, 2, 4); 0: tuned */
+    int waves_per_cu;       /* device step's persistent launch: resident waves per CU; 0: tuned */
+    int use_graph;          /* in: ask for the device step; out: whether it runs (0 in this build) */
     double msps_measured;   /* out: trellis Msamples/s of the tuned launch shape (trial) */
 } gpu_hip_tune_t;
 
@@ -403,6 +464,9 @@ typedef struct {
     size_t seg_len;
     int reposition;
     int fma_o8;                 /* order 8: the AVX2 build's FMA filter */
+    int seeding;                /* DAS seeding: 0 the reference's rule by fs_out (seg0-first
+                                   below DSD512, src/dsp_plugin.c:2185), 1 seg0-first,
+                                   2 every segment from the persistent state */
 } dsdhip_stream_cfg_t;
 dsdhip_stream_t *dsdhip_stream_create_cfg(const dsdhip_stream_cfg_t *cfg);
 /* volume: the next chunk ramps from the previous gain to this one */
@@ -412,9 +476,9 @@ int dsdhip_stream_set_gain(dsdhip_stream_t *s, double gain);
 int dsdhip_stream_process_dsd_dev(dsdhip_stream_t *s, const unsigned long long *d_in, int64_t words_stride,
                                   size_t n, unsigned long long *d_bits, int64_t out_words, size_t *out_count);
 /* ir == NULL: no convolution.  Otherwise ir is host f64 [nch][ir_stride]
- * and chunk_frames * 2^stages must be a multiple of 8*P.  The DAS runs
- * all segments from the persistent contexts (seg0_first = 0) with segments
- * of seg_len samples. */
+ * and chunk_frames * 2^stages must be a multiple of 8*P.  The DAS seeds by
+ * the reference's rule for fs_out (seeding 0 of dsdhip_stream_cfg_t) with
+ * segments of seg_len samples. */
 dsdhip_stream_t *dsdhip_stream_create(unsigned fs_in, unsigned fs_out, double gain, int nch,
                                       size_t chunk_frames, const double *ir, int64_t ir_len,
                                       int64_t ir_stride, int P, int ntf_index, int depth, int cands,

@@ -156,9 +156,25 @@ void or_das_reposition(or_das_plan_t *p, const double *ch0, size_t F);
  * (each F samples, channel-major).  `persist` holds one context per channel
  * and is updated like the reference (copy-back of the last segment).
  * out receives per channel the assembled +-1 stream; returns its length. */
+/* host threads of or_das_run's phases (default 1; results do not change) */
+void or_das_set_threads(int n);
 size_t or_das_run(const or_das_plan_t *p, or_ctx *persist, int nch,
                   const double *in, size_t F, float *out, int *stitch_pos);
 
+/* Every intermediate of one stitch search: half window, density map (dm
+ * when not NULL, ovl_len entries), peak and its first position, the
+ * candidates found by the scan (scanned) and in total (ncand, the peak-run
+ * candidate last), their position, run, density, largest decoded
+ * difference and quality, and the chosen index. */
+typedef struct {
+    int half_w, peak, peak_pos, scanned, ncand, best;
+    int pos[33], run[33], density[33];
+    double quality[33], max_decoded[33];
+    int *dm;
+} or_stitch_detail_t;
+int or_das_stitch_ex(const float *prev, const float *cur, int ovl_len, int lat, or_stitch_detail_t *det);
+/* Candidate choice of src/dsp_plugin.c:2641-2652. */
+int or_das_pick(int n, const double *quality, const int *density);
 /* Stitch search for one boundary (src/dsp_plugin.c:2452-2640):
  * returns the chosen position in [0, ovl_len). */
 int or_das_stitch(const float *prev_ovl, const float *this_ovl, int ovl_len,

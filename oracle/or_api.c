@@ -136,12 +136,56 @@ long ora_das(int filter_index, int depth, int cands, int lat, const double *in, 
     return (long)n;
 }
 
+void ora_das_set_threads(int n) { or_das_set_threads(n); }
+
 int ora_das_nseg(size_t F, size_t seg_len, int nseg, int lat)
 {
     or_das_plan_t p;
     int S = or_das_plan_init(&p, F, seg_len, nseg, (size_t)64 * (size_t)lat, 1, 0);
     or_das_plan_free(&p);
     return S;
+}
+
+/* intermediates of one stitch search: ints[0..5] = half_w, peak,
+ * peak_pos, scanned, ncand, best; cand_i[3][33] = pos, run, density;
+ * cand_d[2][33] = quality, largest decoded difference; dm[ovl] */
+int ora_stitch_detail(const float *prev, const float *cur, int ovl, int lat, int *ints, int *cand_i,
+                      double *cand_d, int *dm)
+{
+    or_stitch_detail_t det;
+    memset(&det, 0, sizeof(det));
+    det.dm = dm;
+    const int pos = or_das_stitch_ex(prev, cur, ovl, lat, &det);
+    ints[0] = det.half_w;
+    ints[1] = det.peak;
+    ints[2] = det.peak_pos;
+    ints[3] = det.scanned;
+    ints[4] = det.ncand;
+    ints[5] = det.best;
+    for (int i = 0; i < det.ncand; i++) {
+        cand_i[i] = det.pos[i];
+        cand_i[33 + i] = det.run[i];
+        cand_i[66 + i] = det.density[i];
+        cand_d[i] = det.quality[i];
+        cand_d[33 + i] = det.max_decoded[i];
+    }
+    return pos;
+}
+
+int ora_pick(int n, const double *quality, const int *density) { return or_das_pick(n, quality, density); }
+
+/* reposition of a plan over channel-0 data: starts/sizes in/out */
+void ora_reposition(size_t F, size_t seg_len, int lat, const double *ch0, long *start, long *size, int *nseg)
+{
+    or_das_plan_t p;
+    int S = or_das_plan_init(&p, F, seg_len, 0, (size_t)64 * (size_t)lat, 1, 1);
+    or_das_reposition(&p, ch0, F);
+    for (int s = 0; s < S; s++) {
+        start[s] = (long)p.nom_start[s];
+        size[s] = (long)p.nom_size[s];
+    }
+    *nseg = S;
+    or_das_plan_free(&p);
 }
 
 int ora_stitch(const float *prev, const float *cur, int ovl, int lat, double *q, int *nc)
@@ -192,6 +236,18 @@ void *ora_das_stream_new(int filter_index, int depth, int cands, int lat, int nc
             return NULL;
         }
     return h;
+}
+
+long ora_das_stream_chunk_n(void *hv, const double *in, size_t F, size_t seg_len, int nseg, int seg0_first,
+                            int reposition, float *out)
+{
+    ora_das_stream_t *h = (ora_das_stream_t *)hv;
+    or_das_plan_t p;
+    or_das_plan_init(&p, F, seg_len, nseg, (size_t)64 * (size_t)h->lat, seg0_first, reposition);
+    if (reposition) or_das_reposition(&p, in, F);
+    size_t n = or_das_run(&p, h->pc, h->nch, in, F, out, NULL);
+    or_das_plan_free(&p);
+    return (long)n;
 }
 
 long ora_das_stream_chunk(void *hv, const double *in, size_t F, size_t seg_len, int seg0_first,

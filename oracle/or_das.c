@@ -20,6 +20,7 @@
  */
 #include "dsd_oracle.h"
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -156,8 +157,21 @@ const double *or_das_qfir(void)
 
 typedef struct { int pos, run, density; double quality; } qcand_t;
 
-int or_das_stitch(const float *prev, const float *cur, int ovl_len, int lat,
-                  double *quality_out, int *ncand_out)
+/* Candidate choice (src/dsp_plugin.c:2641-2652): keep the first; a later
+ * candidate replaces the kept one when its quality is below 0.8 of the
+ * kept one's, or below 1.2 of it with a strictly higher density. */
+int or_das_pick(int n, const double *quality, const int *density)
+{
+    int best = 0;
+    for (int ci = 1; ci < n; ci++) {
+        double rq = quality[ci] / (quality[best] + 1e-15);
+        if (rq < 0.8) best = ci;
+        else if (rq < 1.2 && density[ci] > density[best]) best = ci;
+    }
+    return best;
+}
+
+int or_das_stitch_ex(const float *prev, const float *cur, int ovl_len, int lat, or_stitch_detail_t *det)
 {
     int half_w = lat;
     if (half_w > ovl_len / 2) half_w = ovl_len / 2;
@@ -197,10 +211,12 @@ int or_das_stitch(const float *prev, const float *cur, int ovl_len, int lat,
         q[nq].quality = 1e30;
         nq++;
     }
+    int scanned = nq, peak_pos = 0;
     {
         int pp = 0;
         for (int p = 1; p < ovl_len; p++)
             if (dm[p] > dm[pp]) pp = p;
+        peak_pos = pp;
         int prp = pp, prl = 1;
         for (int mr = 8; mr >= 1 && prl < 4; mr >>= 1) {
             for (int r = 0; r <= half_w && prl < 4; r++) {
@@ -232,6 +248,7 @@ int or_das_stitch(const float *prev, const float *cur, int ovl_len, int lat,
         q[0].pos = 0; q[0].run = 1; q[0].density = 0; q[0].quality = 0;
         nq = 1;
     }
+    double maxd[32] = {0};
     if (nq == 1) q[0].quality = 0;
     if (nq > 1) {
         const double *qf = or_das_qfir();
@@ -251,24 +268,179 @@ int or_das_stitch(const float *prev, const float *cur, int ovl_len, int lat,
                 double ad = fabs(val);
                 if (ad > max_d) max_d = ad;
             }
+            maxd[ci] = max_d;
             double dr = (double)q[ci].density / (double)(peak > 0 ? peak : 1);
             if (dr < 0.5) dr = 0.5;
             q[ci].quality = max_d / dr;
         }
     }
-    int best = 0;
-    for (int ci = 1; ci < nq; ci++) {
-        double rq = q[ci].quality / (q[best].quality + 1e-15);
-        if (rq < 0.8) best = ci;
-        else if (rq < 1.2 && q[ci].density > q[best].density) best = ci;
+    double qual[32];
+    int den[32];
+    for (int ci = 0; ci < nq; ci++) {
+        qual[ci] = q[ci].quality;
+        den[ci] = q[ci].density;
+    }
+    const int best = or_das_pick(nq, qual, den);
+    if (det) {
+        det->half_w = half_w;
+        det->peak = peak;
+        det->peak_pos = peak_pos;
+        det->scanned = scanned;
+        det->ncand = nq;
+        det->best = best;
+        for (int ci = 0; ci < nq; ci++) {
+            det->pos[ci] = q[ci].pos;
+            det->run[ci] = q[ci].run;
+            det->density[ci] = q[ci].density;
+            det->quality[ci] = q[ci].quality;
+            det->max_decoded[ci] = maxd[ci];
+        }
+        if (det->dm)
+            for (int p = 0; p < ovl_len; p++) det->dm[p] = dm[p];
     }
     free(dm);
-    if (quality_out) *quality_out = q[best].quality;
-    if (ncand_out) *ncand_out = nq;
     return q[best].pos;
 }
 
+int or_das_stitch(const float *prev, const float *cur, int ovl_len, int lat,
+                  double *quality_out, int *ncand_out)
+{
+    or_stitch_detail_t det;
+    memset(&det, 0, sizeof(det));
+    const int pos = or_das_stitch_ex(prev, cur, ovl_len, lat, &det);
+    if (quality_out) *quality_out = det.quality[det.best];
+    if (ncand_out) *ncand_out = det.ncand;
+    return pos;
+}
+
 /* ---- the pipeline ---- */
+
+/* Host threads for or_das_run's phases (the reference's threadpool runs
+ * the blocks of one phase concurrently and waits between phases,
+ * src/dsp_plugin.c:2185-2369).  Results do not depend on the count. */
+static int g_das_threads = 1;
+
+void or_das_set_threads(int n) { g_das_threads = n < 1 ? 1 : n; }
+
+typedef struct {
+    void (*fn)(void *, int);
+    void *arg;
+    int n;
+    int next;
+    pthread_mutex_t mu;
+} or_par_t;
+
+static void *par_worker(void *v)
+{
+    or_par_t *p = (or_par_t *)v;
+    for (;;) {
+        pthread_mutex_lock(&p->mu);
+        const int i = p->next++;
+        pthread_mutex_unlock(&p->mu);
+        if (i >= p->n) break;
+        p->fn(p->arg, i);
+    }
+    return NULL;
+}
+
+static void par_for(int n, void (*fn)(void *, int), void *arg)
+{
+    int nt = g_das_threads < n ? g_das_threads : n;
+    if (nt <= 1) {
+        for (int i = 0; i < n; i++) fn(arg, i);
+        return;
+    }
+    or_par_t p = {fn, arg, n, 0, PTHREAD_MUTEX_INITIALIZER};
+    pthread_t th[256];
+    if (nt > 256) nt = 256;
+    for (int t = 0; t < nt; t++) pthread_create(&th[t], NULL, par_worker, &p);
+    for (int t = 0; t < nt; t++) pthread_join(th[t], NULL);
+}
+
+typedef struct {
+    const or_das_plan_t *p;
+    or_ctx *persist, *tmp;
+    int nch, S, temps;
+    const double *in;
+    size_t F;
+    float **sb;
+    size_t *so;
+    double *est;
+} das_job_t;
+
+static void task_seg0(void *v, int ch)
+{
+    das_job_t *j = (das_job_t *)v;
+    const or_das_plan_t *p = j->p;
+    size_t cnt = p->nom_size[0];
+    if (!p->seg0_first && j->S > 1 && p->nom_size[0] + p->overlap <= j->F) cnt += p->overlap;
+    j->so[ch * j->S] = or_sdm_segment(&j->persist[ch], j->in + (size_t)ch * j->F, j->sb[ch * j->S], cnt, 0);
+}
+
+/* seg0-first: seg1 on a copy of the persistent context (after seg0), or
+ * the estimate for seg >= 2 (task index nch + ch) */
+static void task_seg1_or_estimate(void *v, int i)
+{
+    das_job_t *j = (das_job_t *)v;
+    const or_das_plan_t *p = j->p;
+    const int S = j->S;
+    const size_t O = p->overlap, F = j->F;
+    if (i < j->nch) {
+        const int ch = i;
+        or_ctx *t = &j->tmp[ch * j->temps];
+        or_sdm_copy_state(t, &j->persist[ch]);
+        size_t ns = p->nom_start[1], nz = p->nom_size[1], cnt = nz;
+        if (S > 2 && ns + nz + O <= F) cnt += O;
+        j->so[ch * S + 1] = or_sdm_segment(t, j->in + (size_t)ch * F + ns, j->sb[ch * S + 1], cnt, 0);
+        return;
+    }
+    const int ch = i - j->nch;
+    double st[OR_MAX_ORDER];
+    or_sdm_best_state(&j->persist[ch], st);
+    const size_t es = p->nom_start[1];
+    const double *data = j->in + (size_t)ch * F + es;
+    const size_t cnt = F - es;
+    size_t pos = 0;
+    for (int seg = 1; seg < S; seg++) {
+        size_t bnd = seg >= 2 ? p->nom_start[seg] - es : 0;
+        if (bnd > pos && bnd <= cnt) {
+            or_estimate_state(j->persist[ch].f, st, data + pos, bnd - pos, j->persist[ch].state_limit, 16, st);
+            pos = bnd;
+        }
+        memcpy(&j->est[(ch * S + seg) * OR_MAX_ORDER], st, sizeof(st));
+    }
+}
+
+/* segment first + k (k over channels x segments) on a copy of the
+ * persistent context as it is now, seeded with its estimate */
+static void task_segment(void *v, int k)
+{
+    das_job_t *j = (das_job_t *)v;
+    const or_das_plan_t *p = j->p;
+    const int first = p->seg0_first ? 2 : 1;
+    const int per = j->S - first;
+    const int ch = k / per, seg = first + k % per;
+    const size_t O = p->overlap, F = j->F;
+    size_t ns = p->nom_start[seg], nz = p->nom_size[seg];
+    int ext = seg < j->S - 1;
+    size_t is = ns >= O ? ns - O : 0;
+    size_t cnt = ns + nz - is;
+    if (ext && ns + nz + O <= F) cnt += O;
+    size_t disc = ns >= O ? O - j->persist[ch].L : 0;
+    or_ctx *t = &j->tmp[ch * j->temps + seg - 1];
+    or_sdm_copy_state(t, &j->persist[ch]);
+    const double *e = &j->est[(ch * j->S + seg) * OR_MAX_ORDER];
+    int bank = t->idx & 1;
+    for (unsigned c = 0; c < t->num_cands; c++) {
+        or_node *nd = t->act[bank][c];
+        if (nd) {
+            for (int q = 0; q < t->f->order; q++) nd->st[q] = e[q];
+            nd->cost = 0.0;
+        }
+    }
+    j->so[ch * j->S + seg] = or_sdm_segment(t, j->in + (size_t)ch * F + is, j->sb[ch * j->S + seg], cnt, disc);
+}
+
 size_t or_das_run(const or_das_plan_t *p, or_ctx *persist, int nch,
                   const double *in, size_t F, float *out, int *stitch_pos)
 {
@@ -284,39 +456,13 @@ size_t or_das_run(const or_das_plan_t *p, or_ctx *persist, int nch,
         int seg = i % S;
         sb[i] = (float *)malloc((p->nom_size[seg] + 2 * O + 4096) * sizeof(float));
     }
+    das_job_t job = {p, persist, tmp, nch, S, temps, in, F, sb, so, est};
 
     if (seg0_first) {
-        for (int ch = 0; ch < nch; ch++)
-            so[ch * S] = or_sdm_segment(&persist[ch], in + (size_t)ch * F, sb[ch * S],
-                                        p->nom_size[0], 0);
-        if (S > 1) {
-            for (int ch = 0; ch < nch; ch++) {
-                or_ctx *t = &tmp[ch * temps];
-                or_sdm_copy_state(t, &persist[ch]);
-                size_t ns = p->nom_start[1], nz = p->nom_size[1], cnt = nz;
-                if (S > 2 && ns + nz + O <= F) cnt += O;
-                so[ch * S + 1] = or_sdm_segment(t, in + (size_t)ch * F + ns, sb[ch * S + 1], cnt, 0);
-            }
-        }
-        if (S > 2) {
-            for (int ch = 0; ch < nch; ch++) {
-                double st[OR_MAX_ORDER];
-                or_sdm_best_state(&persist[ch], st);
-                const size_t es = p->nom_start[1];
-                const double *data = in + (size_t)ch * F + es;
-                const size_t cnt = F - es;
-                size_t pos = 0;
-                for (int seg = 1; seg < S; seg++) {
-                    size_t bnd = seg >= 2 ? p->nom_start[seg] - es : 0;
-                    if (bnd > pos && bnd <= cnt) {
-                        or_estimate_state(persist[ch].f, st, data + pos, bnd - pos,
-                                          persist[ch].state_limit, 16, st);
-                        pos = bnd;
-                    }
-                    memcpy(&est[(ch * S + seg) * OR_MAX_ORDER], st, sizeof(st));
-                }
-            }
-        }
+        par_for(nch, task_seg0, &job);
+        /* seg1 (if any) and the estimate for seg >= 2 (if any), concurrently */
+        if (S > 2) par_for(2 * nch, task_seg1_or_estimate, &job);
+        else if (S > 1) par_for(nch, task_seg1_or_estimate, &job);
     } else {
         for (int ch = 0; ch < nch; ch++) {
             double st[OR_MAX_ORDER];
@@ -328,35 +474,8 @@ size_t or_das_run(const or_das_plan_t *p, or_ctx *persist, int nch,
 
     /* parallel segments (copies of persistent as it is now) */
     const int first = seg0_first ? 2 : 1;
-    for (int ch = 0; ch < nch; ch++) {
-        for (int seg = first; seg < S; seg++) {
-            size_t ns = p->nom_start[seg], nz = p->nom_size[seg];
-            int ext = seg < S - 1;
-            size_t is = ns >= O ? ns - O : 0;
-            size_t cnt = ns + nz - is;
-            if (ext && ns + nz + O <= F) cnt += O;
-            size_t disc = ns >= O ? O - persist[ch].L : 0;
-            or_ctx *t = &tmp[ch * temps + seg - 1];
-            or_sdm_copy_state(t, &persist[ch]);
-            const double *e = &est[(ch * S + seg) * OR_MAX_ORDER];
-            int bank = t->idx & 1;
-            for (unsigned c = 0; c < t->num_cands; c++) {
-                or_node *nd = t->act[bank][c];
-                if (nd) {
-                    for (int k = 0; k < t->f->order; k++) nd->st[k] = e[k];
-                    nd->cost = 0.0;
-                }
-            }
-            so[ch * S + seg] = or_sdm_segment(t, in + (size_t)ch * F + is, sb[ch * S + seg], cnt, disc);
-        }
-    }
-    if (!seg0_first) {
-        for (int ch = 0; ch < nch; ch++) {
-            size_t cnt = p->nom_size[0];
-            if (S > 1 && p->nom_size[0] + O <= F) cnt += O;
-            so[ch * S] = or_sdm_segment(&persist[ch], in + (size_t)ch * F, sb[ch * S], cnt, 0);
-        }
-    }
+    if (S > first) par_for(nch * (S - first), task_segment, &job);
+    if (!seg0_first) par_for(nch, task_seg0, &job);
 
     /* stitch positions from channel 0 */
     int *sp = (int *)calloc((size_t)S, sizeof(int));

@@ -472,12 +472,16 @@ class DasStream:
         self.nch = nch
         self.h = L.ora_das_stream_new(filter_index, depth, cands, lat, nch)
 
-    def chunk(self, x, seg_len, seg0_first=0, reposition=1):
+    def chunk(self, x, seg_len, seg0_first=0, reposition=1, nseg=0):
+        """seg_len > 0: segments of that length; seg_len == 0: nseg parts"""
         x = np.ascontiguousarray(x, dtype=np.float64)
         F = x.shape[1]
         out = np.zeros((self.nch, F), np.float32)
-        n = self.L.ora_das_stream_chunk(self.h, _ptr(x, _dp), F, seg_len, seg0_first, reposition,
-                                        _ptr(out, _fp))
+        self.L.ora_das_stream_chunk_n.restype = C.c_long
+        self.L.ora_das_stream_chunk_n.argtypes = [C.c_void_p, _dp, C.c_size_t, C.c_size_t, C.c_int, C.c_int,
+                                                  C.c_int, _fp]
+        n = self.L.ora_das_stream_chunk_n(self.h, _ptr(x, _dp), F, seg_len, nseg, seg0_first, reposition,
+                                          _ptr(out, _fp))
         return out[:, :n]
 
     def __del__(self):

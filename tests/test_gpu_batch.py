@@ -38,7 +38,8 @@ def oracle_bits(oracle, pcm, fi, seg_len):
     parts = []
     for f0 in range(0, n, CF):
         f1 = min(n, f0 + CF)
-        parts.append(st.chunk(y[:, f0 * R:f1 * R], seg_len, 0, 1))
+        # DSD128 is below DSD512: the reference's seg0-first seeding
+        parts.append(st.chunk(y[:, f0 * R:f1 * R], seg_len, 1, 1))
     return np.concatenate(parts, axis=1)
 
 

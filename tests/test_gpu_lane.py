@@ -146,6 +146,6 @@ def test_lane_same_rate_reencode_stream(gpu, oracle):
         sdm_in = np.stack([conds[c].dsd_boxcar(src[c, s0:s1], fg, fg) for c in range(nch)])
         sig = S.signal()
         assert np.array_equal(sig, sdm_in), s0
-        ref = das.chunk(sig, seg, 0, 1)
+        ref = das.chunk(sig, seg, int(S.seg0_first), 1)
         assert n == ref.shape[1], s0
         assert np.array_equal(S.bits_numpy(n), ref), s0

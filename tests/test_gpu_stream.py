@@ -3,7 +3,8 @@ SDM input of every chunk against the oracle's FP64 FIR chain followed by an
 FP64 linear convolution of the whole stream (the convolution runs on complex
 FP32 spectra: relative error bound stated below), and the DSD bits against
 the oracle's per-chunk DAS with persistent contexts run on the same SDM
-input (bit-exact)."""
+input (bit-exact), seeded as the reference seeds at the output rate
+(seg0-first below DSD512)."""
 import numpy as np
 import pytest
 import torch
@@ -51,7 +52,7 @@ def test_stream_matches_oracle(gpu, oracle, with_ir):
         ref = y[:, f0 * R:f1 * R]
         err = np.max(np.abs(sig - ref)) / np.max(np.abs(y))
         assert err < (2e-5 if with_ir else 1e-9), (f0, err)
-        want = das.chunk(sig, SEG, 0, 1)
+        want = das.chunk(sig, SEG, int(S.seg0_first), 1)
         assert n == want.shape[1], (f0, n, want.shape)
         assert np.array_equal(S.bits_numpy(n), want), "chunk at frame %d" % f0
         total += n
@@ -84,7 +85,7 @@ def check_stream(S, das, sigs_ref, feed, nch):
         assert sig.shape == ref.shape, k
         bad = np.argwhere(sig != ref)
         assert bad.size == 0, (k, bad[:3], sig[tuple(bad[0])], ref[tuple(bad[0])])
-        want = das.chunk(sig, SEG, 0, 1)
+        want = das.chunk(sig, SEG, int(S.seg0_first), 1)
         assert n == want.shape[1], k
         assert np.array_equal(S.bits_numpy(n), want), k
         total += n
