@@ -42,6 +42,8 @@ _SIGS = {
     "dsdhip_trellis_process": ([_vp, C.c_int, _dp, _fp, C.c_size_t, C.POINTER(C.c_size_t)], C.c_int),
     "dsdhip_trellis_run_jobs": ([_vp, C.c_int, _ip, _ip, C.POINTER(_vp), _i64p, _i64p,
                                  C.POINTER(_vp), _dp, _i64p, _i64p, _ip], C.c_int),
+    "dsdhip_trellis_run_jobs_noalloc": ([_vp, C.c_int, _ip, _ip, C.POINTER(_vp), _i64p, _i64p,
+                                         C.POINTER(_vp), _dp, _i64p, _i64p, _ip], C.c_int),
     "dsdhip_trellis_force_literal": ([_vp, C.c_int], C.c_int),
     "dsdhip_trellis_fma_o8": ([_vp, C.c_int], C.c_int),
     "dsdhip_das_fma_o8": ([_vp, C.c_int], C.c_int),

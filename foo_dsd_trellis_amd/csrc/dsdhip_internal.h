@@ -29,6 +29,9 @@ int dsdhip_trellis_table_launch(dsdhip_trellis_t *t, int first, int n, void *str
 int dsdhip_trellis_table_end(dsdhip_trellis_t *t, void *stream, int64_t *kept, int64_t *slow_steps,
                              int *err_out);
 int dsdhip_trellis_timing_add(dsdhip_trellis_t *t, double ms, int64_t launches);
+int dsdhip_trellis_fill_job(const dsdhip_trellis_t *t, tr_job *jb, int ctx, int src, const double *d_in,
+                            int64_t n_in, int64_t keep_from, uint64_t *d_out, const double *seed10);
+int dsdhip_trellis_set_snap(dsdhip_trellis_t *t, int i, int src);
 /* 255-tap Kaiser low-pass of the stitch quality (src/dsp_plugin.c:2571-2604). */
 void dsdhip_make_qfir(double *q);
 /* Device scratch of the FIR chain for n input samples per channel, so later

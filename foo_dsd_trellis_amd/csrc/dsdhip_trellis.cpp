@@ -416,6 +416,22 @@ int dsdhip_trellis_table_end(dsdhip_trellis_t *t, void *stream, int64_t *kept, i
     return err ? -1 : 0;
 }
 
+/* fill_job for other translation units */
+int dsdhip_trellis_fill_job(const dsdhip_trellis_t *t, tr_job *jb, int ctx, int src, const double *d_in,
+                            int64_t n_in, int64_t keep_from, uint64_t *d_out, const double *seed10)
+{
+    if (!t || !jb) return -1;
+    return fill_job(t, *jb, ctx, src, d_in, n_in, keep_from, d_out, seed10);
+}
+
+/* records that context i now follows links into context src */
+int dsdhip_trellis_set_snap(dsdhip_trellis_t *t, int i, int src)
+{
+    if (!t || i < 0 || i >= t->nctx) return -1;
+    t->snap_of[i] = src;
+    return 0;
+}
+
 /* Adds a measured span to the launch timing (pipelined runs time their
  * trellis work as one span). */
 int dsdhip_trellis_timing_add(dsdhip_trellis_t *t, double ms, int64_t launches)

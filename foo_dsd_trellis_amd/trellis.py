@@ -101,12 +101,14 @@ class SdmPool:
         check(lib().dsdhip_trellis_drain(self.h, i, out.ctypes.data_as(_fp), max_out, C.byref(cnt)), "drain")
         return out[:cnt.value]
 
-    def run_jobs(self, ctx, d_in, n_in, d_out, keep_from=None, copy_src=None, seed=None):
+    def run_jobs(self, ctx, d_in, n_in, d_out, keep_from=None, copy_src=None, seed=None, noalloc=False):
         """Batched launch on device buffers.
 
         ctx, n_in, keep_from, copy_src: int sequences (one per job);
         d_in / d_out: device addresses (ints) of float64 input and uint64
         packed-bit output; seed: (njobs, 10) float64, NaN row = no seed.
+        noalloc: dsdhip_trellis_run_jobs_noalloc (job table in a static
+        device array, nothing allocated; at most DSDHIP_NOALLOC_MAX_JOBS).
         Returns (kept, slow_steps) arrays.
         """
         nj = len(ctx)
@@ -122,7 +124,8 @@ class SdmPool:
         sd = None
         if seed is not None:
             sd = np.ascontiguousarray(seed, dtype=np.float64).reshape(nj, 10)
-        rc = lib().dsdhip_trellis_run_jobs(
+        fn = lib().dsdhip_trellis_run_jobs_noalloc if noalloc else lib().dsdhip_trellis_run_jobs
+        rc = fn(
             self.h, nj, ci, cs, pin, ni, kf, pout,
             sd.ctypes.data_as(_dp) if sd is not None else None, kept, slow, C.byref(err))
         if rc != 0:

@@ -218,6 +218,18 @@ int dsdhip_trellis_run_jobs(dsdhip_trellis_t *t, int njobs, const int *ctx, cons
                             const int64_t *keep_from, uint64_t *const *d_out,
                             const double *seed, int64_t *kept, int64_t *slow_steps,
                             int *err_out);
+/* dsdhip_trellis_run_jobs that allocates nothing, for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops).
+ * No counterpart in the reference.  The job table lives in a static device
+ * array of DSDHIP_NOALLOC_MAX_JOBS descriptors, one per GPU, held until the
+ * process ends; more jobs are rejected (-1, message on stderr).  Calls that
+ * overlap on one GPU share the table and race. */
+#define DSDHIP_NOALLOC_MAX_JOBS 4096
+int dsdhip_trellis_run_jobs_noalloc(dsdhip_trellis_t *t, int njobs, const int *ctx, const int *copy_src,
+                                    const double *const *d_in, const int64_t *n_in,
+                                    const int64_t *keep_from, uint64_t *const *d_out,
+                                    const double *seed, int64_t *kept, int64_t *slow_steps,
+                                    int *err_out);
 /* Validation switch: decide every step with the literal sequential prune. */
 int dsdhip_trellis_force_literal(dsdhip_trellis_t *t, int on);
 /* Order 8: the AVX2 build's 4-candidate FMA filter (src/trellis.c:348-533,
