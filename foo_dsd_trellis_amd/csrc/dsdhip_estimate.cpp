@@ -70,6 +70,9 @@ struct Quant {
     double lo[32], hi[32], y[32];
     double over;         /* v >= over: conversion overflow, idx 0 */
     double c1, c2;       /* idx ~ trunc(v * c1 + c2) */
+    /* one-step lookahead of the scaled quantizer input (no limiter):
+     * c1 v_{t+1} + c2 ~ c2 + cx x_{t+1} + h1 (x_t - y_t) + r1 . s_{t-1} */
+    double cx, h1, r1[TR_MAXP], hy[32];
 };
 
 void quant_init(Quant &q, double a0, int levels)
@@ -185,6 +188,60 @@ int fast_block(const double *a, const double *g, double *sv, const double *x, si
     return bad;
 }
 
+/* The same without a limiter, with the index of sample t+1 guessed one
+ * step ahead: in exact arithmetic the loop filter is linear, so
+ * c1 v_{t+1} + c2 = c2 + cx x_{t+1} + h1 (x_t - y_t) + r1 . s_{t-1}, and
+ * the chain from one guess to the next is a table read, a subtraction and
+ * the truncation.  The state still advances in the reference's operations
+ * and every guess is checked against the reference-order sum. */
+template <int P>
+int fast_block_la(const double *a, const double *g, double *sv, const double *x, size_t n, const Quant &q)
+{
+    double s[P];
+    for (int i = 0; i < P; i++) s[i] = sv[i];
+    const int top = q.levels - 1;
+    const double c1 = q.c1, c2 = q.c2, over = q.over, cx = q.cx, h1 = q.h1;
+    const double e0 = a[0] * c1, e1 = a[1] * c1, es = e0 + e1;
+    int bad = 0;
+    /* first guess from the start state (the single-step form) */
+    int k;
+    {
+        double ns[P];
+        for (int i = 2; i < P - 1; i++) ns[i] = s[i] + s[i - 1] - g[i] * s[i + 1];
+        ns[P - 1] = s[P - 1] + s[P - 2];
+        double r = x[0] + a[2] * ns[2];
+        for (int i = 3; i < P; i++) r = r + a[i] * ns[i];
+        const double w = (((r * c1 + c2) + e0 * (x[0] - g[0] * s[1])) + e1 * (s[1] - g[1] * s[2])) + es * s[0];
+        k = x86_trunc(w);
+        k = k < 0 ? 0 : (k > top ? top : k);
+    }
+    for (size_t t = 0; t < n; t++) {
+        const double xv = x[t];
+        const double xn = (t + 1 < n) ? x[t + 1] : 0.0;
+        double ns[P], p[P];
+        ns[0] = s[0] - g[0] * s[1] + xv;
+        for (int i = 1; i < P - 1; i++) ns[i] = s[i] + s[i - 1] - g[i] * s[i + 1];
+        ns[P - 1] = s[P - 1] + s[P - 2];
+        for (int i = 0; i < P; i++) p[i] = a[i] * ns[i];
+        double ve = xv + p[0];
+        for (int i = 1; i < P; i++) ve = ve + p[i];
+        const bool in = (ve >= q.lo[k]) & (ve < q.hi[k]);
+        const bool zero_ok = (k == 0) & ((ve >= over) | (ve != ve));
+        bad |= !(in | zero_ok);
+        /* next guess: terms of s_{t-1} other than s0 first, s0's term last */
+        double qn = (c2 + cx * xn) + h1 * xv;
+        for (int i = P - 1; i >= 1; i--) qn = qn + q.r1[i] * s[i];
+        qn = qn + q.r1[0] * s[0];
+        int kn = x86_trunc(qn - q.hy[k]);
+        kn = kn < 0 ? 0 : (kn > top ? top : kn);
+        ns[0] = ns[0] - q.y[k];
+        for (int i = 0; i < P; i++) s[i] = ns[i];
+        k = kn;
+    }
+    for (int i = 0; i < P; i++) sv[i] = s[i];
+    return bad;
+}
+
 template <int P>
 void advance(const double *a, const double *g, double *s, const double *x, size_t n, const Quant &q,
              double lim, int64_t *redo)
@@ -196,7 +253,7 @@ void advance(const double *a, const double *g, double *s, const double *x, size_
             double keep[P];
             for (int i = 0; i < P; i++) keep[i] = s[i];
             const int bad = (lim > 0.0) ? fast_block<P, true>(a, g, s, x + off, m, q, lim)
-                                        : fast_block<P, false>(a, g, s, x + off, m, q, lim);
+                                        : fast_block_la<P>(a, g, s, x + off, m, q);
             if (!bad) continue;
             for (int i = 0; i < P; i++) s[i] = keep[i];
             if (redo) (*redo)++;
@@ -231,6 +288,29 @@ dsdhip_est_t *dsdhip_est_create(const double *a, const double *g, int order, dou
     e->P = order;
     e->lim = state_limit;
     quant_init(e->q, a[0], levels);
+    if (e->q.ok) {
+        /* loop filter without quantizer: ns = M s + e0 x (src/trellis.c:1140-1150) */
+        const int P = order;
+        double M[TR_MAXP][TR_MAXP] = {{0}};
+        M[0][0] = 1.0;
+        M[0][1] = -g[0];
+        for (int i = 1; i < P - 1; i++) {
+            M[i][i - 1] = 1.0;
+            M[i][i] = 1.0;
+            M[i][i + 1] = -g[i];
+        }
+        M[P - 1][P - 2] = 1.0;
+        M[P - 1][P - 1] = 1.0;
+        double aM[TR_MAXP] = {0}, aM2[TR_MAXP] = {0};
+        for (int j = 0; j < P; j++)
+            for (int i = 0; i < P; i++) aM[j] += a[i] * M[i][j];
+        for (int j = 0; j < P; j++)
+            for (int i = 0; i < P; i++) aM2[j] += aM[i] * M[i][j];
+        e->q.cx = e->q.c1 * (1.0 + a[0]);
+        e->q.h1 = e->q.c1 * aM[0];
+        for (int j = 0; j < TR_MAXP; j++) e->q.r1[j] = j < P ? e->q.c1 * aM2[j] : 0.0;
+        for (int k = 0; k < levels; k++) e->q.hy[k] = e->q.h1 * e->q.y[k];
+    }
     for (int i = 0; i < 10; i++) e->s[i] = (init10 && i < order) ? init10[i] : 0.0;
     e->redo = 0;
     return e;
