@@ -110,6 +110,8 @@ _SIGS = {
                            _vp, _vp], C.c_int),
     "dsdhip_lf_backward": ([_vp, _vp, C.c_int, C.c_double, _vp, C.c_int, C.c_size_t, C.c_int, _vp,
                             _vp, _vp, _vp, _vp, _vp, C.c_size_t, C.c_int, _vp], C.c_int),
+    "dsdhip_lf_backward_ex": ([_vp, _vp, C.c_int, C.c_double, _vp, C.c_int, C.c_size_t, C.c_int, _vp,
+                               _vp, _vp, _vp, _vp, _vp, C.c_size_t, C.c_int, C.c_int, _vp], C.c_int),
     "dsdhip_pe_workspace_bytes": ([C.c_int, C.c_size_t, C.c_int], C.c_size_t),
     "dsdhip_pe_forward": ([_vp, C.c_int, _vp, C.c_int, C.c_size_t, _vp, _vp], C.c_int),
     "dsdhip_pe_backward": ([_vp, C.c_int, C.c_size_t, C.c_int, _vp, _vp, _vp, C.c_size_t, _vp],

@@ -15,7 +15,7 @@ int lf_launch_recompute(int P, const double *a, const double *g, double T, const
                         double *S, double *YD, hipStream_t st);
 int lf_launch_adj_seq(int P, const double *a, const double *g, const double *gy, int B,
                       int64_t n_slab, int64_t cnt, const double *YD, double *lam, int first,
-                      int SUB, double *LAMCK, double *VB, hipStream_t st);
+                      int SUB, double *LAMCK, double *VB, int64_t span, hipStream_t st);
 int lf_launch_adj_par(int P, const double *a, const double *g, const double *u, int B,
                       int64_t n_slab, int64_t cnt, int SUB, const double *S, const double *LAMCK,
                       const double *VB, double *gu, double *part, int nblocks, hipStream_t st);
@@ -100,16 +100,17 @@ int dsdhip_lf_forward(const double *d_a, const double *d_g, int order, double te
                          (hipStream_t)stream);
 }
 
-int dsdhip_lf_backward(const double *d_a, const double *d_g, int order, double temperature,
-                       const double *d_u, int B, size_t N, int ck_every, const double *d_ck,
-                       const double *d_gy, double *d_gu, double *d_ga, double *d_gg, void *d_ws,
-                       size_t ws_bytes, int slab_chunks, void *stream)
+int dsdhip_lf_backward_ex(const double *d_a, const double *d_g, int order, double temperature,
+                          const double *d_u, int B, size_t N, int ck_every, const double *d_ck,
+                          const double *d_gy, double *d_gu, double *d_ga, double *d_gg, void *d_ws,
+                          size_t ws_bytes, int slab_chunks, int trunc_span, void *stream)
 {
     if (!lf_args_ok(order, B, N, ck_every) || !d_a || !d_g || !d_u || !d_ck || !d_gy || !d_gu ||
         !d_ga || !d_gg || !d_ws)
         return -1;
     if (dsdhip_ensure_device()) return -1;
     const int P = order, E = ck_every;
+    if (trunc_span < 0 || (trunc_span > 0 && (E % trunc_span != 0 || trunc_span % lf_sub(E) != 0))) return -1;
     const size_t nck = dsdhip_lf_checkpoints(N, E);
     const int G = (slab_chunks <= 0 || (size_t)slab_chunks > nck) ? (int)nck : slab_chunks;
     LfWs w;
@@ -128,7 +129,7 @@ int dsdhip_lf_backward(const double *d_a, const double *d_g, int order, double t
         const int SUB = lf_sub(E);
         const int64_t cnt = n_hi - n_slab;
         if (lf_launch_adj_seq(P, d_a, d_g, d_gy, B, n_slab, cnt, w.YD, w.lam, first, SUB,
-                              w.LAMCK, w.VB, st))
+                              w.LAMCK, w.VB, (int64_t)trunc_span, st))
             return -1;
         const int64_t tasks = ((cnt + SUB - 1) / SUB) * ((B + 63) / 64);
         const int nblk = (int)((tasks + 3) / 4 < kParBlocks ? (tasks + 3) / 4 : kParBlocks);
@@ -143,6 +144,15 @@ int dsdhip_lf_backward(const double *d_a, const double *d_g, int order, double t
         hipSuccess)
         return -1;
     return 0;
+}
+
+int dsdhip_lf_backward(const double *d_a, const double *d_g, int order, double temperature,
+                       const double *d_u, int B, size_t N, int ck_every, const double *d_ck,
+                       const double *d_gy, double *d_gu, double *d_ga, double *d_gg, void *d_ws,
+                       size_t ws_bytes, int slab_chunks, void *stream)
+{
+    return dsdhip_lf_backward_ex(d_a, d_g, order, temperature, d_u, B, N, ck_every, d_ck, d_gy, d_gu, d_ga,
+                                 d_gg, d_ws, ws_bytes, slab_chunks, 0, stream);
 }
 
 size_t dsdhip_pe_workspace_bytes(int B, size_t N, int ntaps)

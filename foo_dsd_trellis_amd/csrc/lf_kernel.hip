@@ -259,6 +259,9 @@ __device__ __forceinline__ void lf_lambda_step(double (&l)[P], double (&nb)[P], 
 // sample and lambda at the top of every SUB-sample sub-chunk (LAMCK[q] =
 // lambda after sample min(cnt, (q+1)SUB) - 1).  lam carries lambda between
 // slab launches ([P][B]).  SUB is a multiple of LF_U.
+// span > 0: truncated adjoint, lambda restarts from zero at the top of
+// every span-sample interval (span a multiple of SUB), so blockIdx.y runs
+// the intervals of the slab independently and lam is not used.
 template <int P>
 __global__ __launch_bounds__(64) void lf_adj_seq(const double *__restrict__ a,
                                                  const double *__restrict__ g,
@@ -267,26 +270,31 @@ __global__ __launch_bounds__(64) void lf_adj_seq(const double *__restrict__ a,
                                                  const double *__restrict__ YD,
                                                  double *__restrict__ lam, int first, int SUB,
                                                  double *__restrict__ LAMCK,
-                                                 double *__restrict__ VB)
+                                                 double *__restrict__ VB, int64_t span)
 {
     const int b = blockIdx.x * 64 + threadIdx.x;
     if (b >= B) return;
+    const int64_t lo = span > 0 ? (int64_t)blockIdx.y * span : 0;
+    if (lo >= cnt) return;
+    const int64_t cn = span > 0 ? (cnt - lo < span ? cnt - lo : span) : cnt;
     LfCoef<P> c;
     lf_load_coef<P>(c, a, g);
     double l[P], nb[P];
 #pragma unroll
-    for (int i = 0; i < P; i++) l[i] = first ? 0.0 : lam[(int64_t)i * B + b];
-    const double *gyp = gy + n_slab * B + b;
+    for (int i = 0; i < P; i++) l[i] = (first || span > 0) ? 0.0 : lam[(int64_t)i * B + b];
+    const double *gyp = gy + (n_slab + lo) * B + b;
+    const double *ydp = YD + lo * B + b;
+    double *vbl = VB + lo * B + b;
     auto store_ck = [&](int64_t m) {
-        double *d = LAMCK + (m / SUB) * P * (int64_t)B + b;
+        double *d = LAMCK + ((lo + m) / SUB) * P * (int64_t)B + b;
 #pragma unroll
         for (int i = 0; i < P; i++) d[(int64_t)i * B] = l[i];
     };
-    store_ck(cnt - 1);
-    const int64_t nfull = cnt / LF_U;
-    for (int64_t m = cnt - 1; m >= nfull * LF_U; m--) {       // partial top block
-        const double vbar = (gyp[m * B] - l[0]) * YD[m * B + b];
-        VB[m * B + b] = vbar;
+    store_ck(cn - 1);
+    const int64_t nfull = cn / LF_U;
+    for (int64_t m = cn - 1; m >= nfull * LF_U; m--) {       // partial top block
+        const double vbar = (gyp[m * B] - l[0]) * ydp[m * B];
+        vbl[m * B] = vbar;
         lf_lambda_step<P>(l, nb, c, vbar);
     }
     double cg[LF_U], cy[LF_U], ng[LF_U], ny[LF_U];
@@ -295,7 +303,7 @@ __global__ __launch_bounds__(64) void lf_adj_seq(const double *__restrict__ a,
 #pragma unroll
         for (int j = 0; j < LF_U; j++) {
             cg[j] = gyp[(m0 + j) * B];
-            cy[j] = YD[(m0 + j) * B + b];
+            cy[j] = ydp[(m0 + j) * B];
         }
     }
     for (int64_t k = nfull - 1; k >= 0; k--) {
@@ -304,12 +312,12 @@ __global__ __launch_bounds__(64) void lf_adj_seq(const double *__restrict__ a,
 #pragma unroll
             for (int j = 0; j < LF_U; j++) {
                 ng[j] = gyp[(m0 + j) * B];
-                ny[j] = YD[(m0 + j) * B + b];
+                ny[j] = ydp[(m0 + j) * B];
             }
         }
         const int64_t mtop = k * LF_U + LF_U - 1;
-        if ((mtop + 1) % SUB == 0) store_ck(mtop);
-        double *vbp = VB + mtop * B + b;
+        if ((lo + mtop + 1) % SUB == 0) store_ck(mtop);
+        double *vbp = vbl + mtop * B;
 #pragma unroll
         for (int j = LF_U - 1; j >= 0; j--) {
             const double vbar = (cg[j] - l[0]) * cy[j];
@@ -323,8 +331,10 @@ __global__ __launch_bounds__(64) void lf_adj_seq(const double *__restrict__ a,
             cy[j] = ny[j];
         }
     }
+    if (span == 0) {
 #pragma unroll
-    for (int i = 0; i < P; i++) lam[(int64_t)i * B + b] = l[i];
+        for (int i = 0; i < P; i++) lam[(int64_t)i * B + b] = l[i];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -526,11 +536,14 @@ int lf_launch_recompute(int P, const double *a, const double *g, double T, const
 
 int lf_launch_adj_seq(int P, const double *a, const double *g, const double *gy, int B,
                       int64_t n_slab, int64_t cnt, const double *YD, double *lam, int first,
-                      int SUB, double *LAMCK, double *VB, hipStream_t st)
+                      int SUB, double *LAMCK, double *VB, int64_t span, hipStream_t st)
 {
-    dim3 grid((B + 63) / 64), blk(64);
+    if (span > 0 && (span % SUB) != 0) return -1;
+    const int64_t nint = span > 0 ? (cnt + span - 1) / span : 1;
+    if (nint > 65535) return -1;
+    dim3 grid((B + 63) / 64, (unsigned)nint), blk(64);
     switch (P) {
-#define LF_CASE(PP) case PP: hipLaunchKernelGGL(lf_adj_seq<PP>, grid, blk, 0, st, a, g, gy, B, n_slab, cnt, YD, lam, first, SUB, LAMCK, VB); break;
+#define LF_CASE(PP) case PP: hipLaunchKernelGGL(lf_adj_seq<PP>, grid, blk, 0, st, a, g, gy, B, n_slab, cnt, YD, lam, first, SUB, LAMCK, VB, span); break;
         LF_CASE(3) LF_CASE(4) LF_CASE(5) LF_CASE(6) LF_CASE(7) LF_CASE(8) LF_CASE(9) LF_CASE(10)
 #undef LF_CASE
     default: return -1;

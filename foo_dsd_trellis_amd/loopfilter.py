@@ -48,7 +48,7 @@ class LoopFilterFunction(torch.autograd.Function):
     float64; returns y as (N, B)."""
 
     @staticmethod
-    def forward(ctx, u_nb, a, g, temperature, checkpoint, slab_chunks):
+    def forward(ctx, u_nb, a, g, temperature, checkpoint, slab_chunks, trunc_span=0):
         L = _lib.lib()
         N, B = u_nb.shape
         P = a.numel()
@@ -61,14 +61,14 @@ class LoopFilterFunction(torch.autograd.Function):
                                        u_nb.data_ptr(), B, N, checkpoint, y.data_ptr(),
                                        ck.data_ptr(), _stream()), "dsdhip_lf_forward")
         ctx.save_for_backward(u_nb, a, g, ck)
-        ctx.cfg = (float(temperature), int(checkpoint), int(slab_chunks))
+        ctx.cfg = (float(temperature), int(checkpoint), int(slab_chunks), int(trunc_span))
         return y
 
     @staticmethod
     def backward(ctx, gy):
         L = _lib.lib()
         u_nb, a, g, ck = ctx.saved_tensors
-        T, E, G = ctx.cfg
+        T, E, G, tr = ctx.cfg
         N, B = u_nb.shape
         P = a.numel()
         gy = gy.to(torch.float64).contiguous()
@@ -77,11 +77,11 @@ class LoopFilterFunction(torch.autograd.Function):
         gu = torch.empty_like(u_nb)
         ga = torch.empty(P, dtype=torch.float64, device=u_nb.device)
         gg = torch.empty(P, dtype=torch.float64, device=u_nb.device)
-        _lib.check(L.dsdhip_lf_backward(a.data_ptr(), g.data_ptr(), P, T, u_nb.data_ptr(), B, N,
-                                        E, ck.data_ptr(), gy.data_ptr(), gu.data_ptr(),
-                                        ga.data_ptr(), gg.data_ptr(), ws.data_ptr(), ws_bytes, G,
-                                        _stream()), "dsdhip_lf_backward")
-        return gu, ga, gg, None, None, None
+        _lib.check(L.dsdhip_lf_backward_ex(a.data_ptr(), g.data_ptr(), P, T, u_nb.data_ptr(), B, N,
+                                           E, ck.data_ptr(), gy.data_ptr(), gu.data_ptr(),
+                                           ga.data_ptr(), gg.data_ptr(), ws.data_ptr(), ws_bytes, G,
+                                           tr, _stream()), "dsdhip_lf_backward_ex")
+        return gu, ga, gg, None, None, None, None
 
 
 class PreEmphasisFunction(torch.autograd.Function):
@@ -140,7 +140,13 @@ class DiffSDM(torch.nn.Module):
     coefficients taken from the reference's table (src/ntf.c)."""
 
     def __init__(self, ntf=None, rate=5644800, temperature=10.0, input_scale=1.0,
-                 checkpoint=DEFAULT_CHECKPOINT, trainable=True, slab_chunks=0, device="cuda"):
+                 checkpoint=DEFAULT_CHECKPOINT, trainable=True, slab_chunks=0, trunc_span=0,
+                 device="cuda"):
+        """trunc_span > 0: backpropagate within intervals of that many
+        samples only (the state at each interval start held constant;
+        trunc_span divides the checkpoint interval and is a multiple of 64):
+        bounded gradients at high temperature, where the full adjoint grows
+        without bound."""
         super().__init__()
         if ntf is None:
             idx = trellis.ntf_auto(rate)
@@ -164,6 +170,7 @@ class DiffSDM(torch.nn.Module):
         self.input_scale = float(input_scale)
         self.checkpoint = int(checkpoint)
         self.slab_chunks = int(slab_chunks)
+        self.trunc_span = int(trunc_span)
 
     @property
     def order(self):
@@ -174,7 +181,7 @@ class DiffSDM(torch.nn.Module):
         if self.input_scale != 1.0:
             x = x * self.input_scale
         y = LoopFilterFunction.apply(_time_major(x), self.a, self.g, self.temperature,
-                                     self.checkpoint, self.slab_chunks)
+                                     self.checkpoint, self.slab_chunks, self.trunc_span)
         return y.t()
 
 

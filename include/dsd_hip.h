@@ -434,6 +434,15 @@ int dsdhip_lf_backward(const double *d_a, const double *d_g, int order, double t
                        const double *d_u, int B, size_t N, int ck_every, const double *d_ck,
                        const double *d_gy, double *d_gu, double *d_ga, double *d_gg, void *d_ws,
                        size_t ws_bytes, int slab_chunks, void *stream);
+/* dsdhip_lf_backward with trunc_span > 0: the adjoint restarts from zero at
+ * the start of every trunc_span-sample interval (the gradient with the state
+ * at each interval start held constant, truncated backpropagation through
+ * time); the intervals then run in parallel.  trunc_span divides ck_every
+ * and is a multiple of gcd(ck_every, 64); 0 is dsdhip_lf_backward. */
+int dsdhip_lf_backward_ex(const double *d_a, const double *d_g, int order, double temperature,
+                          const double *d_u, int B, size_t N, int ck_every, const double *d_ck,
+                          const double *d_gy, double *d_gu, double *d_ga, double *d_gg, void *d_ws,
+                          size_t ws_bytes, int slab_chunks, int trunc_span, void *stream);
 /* Pre-emphasis: u[n][b] = sum_k w[k] p[b][n-k] (p row-major [B][N] float32,
  * zero history, 1 <= ntaps <= 128). */
 size_t dsdhip_pe_workspace_bytes(int B, size_t N, int ntaps);

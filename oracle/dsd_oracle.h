@@ -187,6 +187,8 @@ void or_lf_forward(const double *a, const double *g, int P, double T, const doub
                    size_t N, double *y);
 void or_lf_backward(const double *a, const double *g, int P, double T, const double *u, int B,
                     size_t N, const double *gy, double *gu, double *ga, double *gg);
+void or_lf_backward_tr(const double *a, const double *g, int P, double T, const double *u, int B,
+                       size_t N, const double *gy, double *gu, double *ga, double *gg, size_t trunc_every);
 void or_pe_forward(const double *w, int T, const float *p, int B, size_t N, double *u);
 void or_pe_backward(int T, const float *p, int B, size_t N, const double *gu, double *gw);
 

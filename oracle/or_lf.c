@@ -95,6 +95,15 @@ void or_lf_forward(const double *a, const double *g, int P, double T, const doub
 void or_lf_backward(const double *a, const double *g, int P, double T, const double *u, int B,
                     size_t N, const double *gy, double *gu, double *ga, double *gg)
 {
+    or_lf_backward_tr(a, g, P, T, u, B, N, gy, gu, ga, gg, 0);
+}
+
+/* trunc_every > 0: the adjoint restarts from zero at the top of every
+ * trunc_every-sample interval (the state at each interval start held
+ * constant, truncated backpropagation through time) */
+void or_lf_backward_tr(const double *a, const double *g, int P, double T, const double *u, int B,
+                       size_t N, const double *gy, double *gu, double *ga, double *gg, size_t trunc_every)
+{
     double *S = (double *)malloc(sizeof(double) * (size_t)P * (N ? N : 1));
     double *NS = (double *)malloc(sizeof(double) * (size_t)P * (N ? N : 1));
     double *Y = (double *)malloc(sizeof(double) * (N ? N : 1));
@@ -108,6 +117,8 @@ void or_lf_backward(const double *a, const double *g, int P, double T, const dou
         }
         double lam[OR_MAX_ORDER] = {0}, nb[OR_MAX_ORDER];
         for (size_t m = N; m-- > 0;) {
+            if (trunc_every && (m + 1) % trunc_every == 0)
+                for (int i = 0; i < P; i++) lam[i] = 0.0;
             const double *sn = S + m * P, *nsn = NS + m * P;
             double ybar = gy[(size_t)b * N + m] - lam[0];
             double vbar = ybar * ((1.0 - Y[m] * Y[m]) * T);

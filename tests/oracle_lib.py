@@ -410,11 +410,12 @@ def lf_forward(lib_, a, g, T, u):
     return y
 
 
-def lf_backward(lib_, a, g, T, u, gy):
-    """Oracle adjoint: returns (gu (B, N), ga (P,), gg (P,))."""
+def lf_backward(lib_, a, g, T, u, gy, trunc_every=0):
+    """Oracle adjoint: returns (gu (B, N), ga (P,), gg (P,)); trunc_every > 0
+    restarts the adjoint from zero every trunc_every samples."""
     L = lib_.lib
-    L.or_lf_backward.argtypes = [_dp, _dp, C.c_int, C.c_double, _dp, C.c_int, C.c_size_t, _dp, _dp,
-                                 _dp, _dp]
+    L.or_lf_backward_tr.argtypes = [_dp, _dp, C.c_int, C.c_double, _dp, C.c_int, C.c_size_t, _dp, _dp,
+                                    _dp, _dp, C.c_size_t]
     a = np.ascontiguousarray(a, np.float64)
     g = np.ascontiguousarray(g, np.float64)
     u = np.ascontiguousarray(u, np.float64)
@@ -422,8 +423,8 @@ def lf_backward(lib_, a, g, T, u, gy):
     gu = np.zeros_like(u)
     ga = np.zeros(a.size)
     gg = np.zeros(a.size)
-    L.or_lf_backward(_ptr(a, _dp), _ptr(g, _dp), a.size, T, _ptr(u, _dp), u.shape[0], u.shape[1],
-                     _ptr(gy, _dp), _ptr(gu, _dp), _ptr(ga, _dp), _ptr(gg, _dp))
+    L.or_lf_backward_tr(_ptr(a, _dp), _ptr(g, _dp), a.size, T, _ptr(u, _dp), u.shape[0], u.shape[1],
+                        _ptr(gy, _dp), _ptr(gu, _dp), _ptr(ga, _dp), _ptr(gg, _dp), trunc_every)
     return gu, ga, gg
 
 

@@ -89,3 +89,29 @@ def test_full_size_gradient_vs_fp64_oracle(gpu, oracle):
             rel(pre.taps.grad.cpu().numpy(), gw))
     print("full-size gradient rel err (a, g, taps):", errs)
     assert max(errs) < 1e-4
+
+
+@pytest.mark.parametrize("T,N,E,B,G,H", [(10.0, 5000, 512, 3, 3, 64), (10.0, 2 * 4096 + 300, 4096, 4, 0, 64),
+                                         (1.0, 5000, 512, 66, 0, 512), (10.0, 4096, 4096, 2, 0, 128)])
+def test_truncated_gradients_match_oracle(gpu, oracle, T, N, E, B, G, H):
+    """Adjoint restarted every H samples (intervals in parallel) against the
+    FP64 oracle's truncated adjoint: finite at temperature 10, where the full
+    adjoint overflows over these lengths."""
+    from foo_dsd_trellis_amd import loopfilter as LF
+    torch.manual_seed(1)
+    p = signal(B, N, 6)
+    pre = LF.FixedPreEmphasis(63)
+    with torch.no_grad():
+        pre.taps[2:] = 0.003 * torch.randn(61, dtype=torch.float64, device="cuda")
+    sdm = LF.DiffSDM(rate=DSD128, temperature=T, checkpoint=E, slab_chunks=G, trunc_span=H)
+    LF.train_step(pre, sdm, LF.AudioBandLoss(DSD128), torch.from_numpy(p).cuda())
+    w = pre.taps.detach().cpu().numpy()
+    a, g = ntf(oracle)
+    u = ol.pe_forward(oracle, w, p)
+    y = ol.lf_forward(oracle, a, g, T, u)
+    gu, ga, gg = ol.lf_backward(oracle, a, g, T, u, inband_grad_np(y, p.astype(np.float64), DSD128), H)
+    gw = ol.pe_backward(oracle, 63, p, gu)
+    assert np.isfinite(ga).all() and np.isfinite(gw).all()
+    assert rel(sdm.a.grad.cpu().numpy(), ga) < 1e-9
+    assert rel(sdm.g.grad.cpu().numpy(), gg) < 1e-9
+    assert rel(pre.taps.grad.cpu().numpy(), gw) < 1e-9
