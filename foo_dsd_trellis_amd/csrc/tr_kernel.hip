@@ -749,6 +749,8 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
             co->cost[myr] = rcost;
             co->path[myr] = rpath;
             co->next[myr] = (uint8_t)rnext;
+            /* after a step the survivor sits in its child's lane, 2*parent + branch */
+            co->slot[myr] = n_in > 0 ? (uint8_t)lane : c0->slot[lane];
         }
         bool fr = false;
         for (int b = 0; b < 2; b++) {

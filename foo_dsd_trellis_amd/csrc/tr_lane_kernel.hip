@@ -83,6 +83,7 @@ __device__ void tr_lane_body(const tr_params &prm, tr_job *jb, tr_ctx *ctxs)
     double cs0, cs1;
     uint32_t pa0, pa1;
     int nx0, nx1;
+    int sl0, sl1;               /* slots of the survivors in the active bank */
     u64 h0[HW], h1[HW];
     {
         const bool sd = jb->seed != 0;
@@ -98,6 +99,8 @@ __device__ void tr_lane_body(const tr_params &prm, tr_job *jb, tr_ctx *ctxs)
         pa1 = two ? c0->path[1] : 0u;
         nx0 = c0->next[0];
         nx1 = two ? c0->next[1] : 0;
+        sl0 = c0->slot[0];
+        sl1 = two ? c0->slot[1] : 0;
 #pragma unroll
         for (int w = 0; w < HW; w++) {
             h0[w] = c0->hist[0][w];
@@ -262,6 +265,8 @@ __device__ void tr_lane_body(const tr_params &prm, tr_job *jb, tr_ctx *ctxs)
             cs1 = __longlong_as_double(ab1) - minc;
             nx0 = q0 ? nn1 : nn0;
             nx1 = q1 ? nn1 : nn0;
+            sl0 = a0s;                /* child i of a step is slot i: 2*parent + branch */
+            sl1 = a1s;
 #pragma unroll
             for (int q = 0; q < P; q++) {
                 st0[q] = ns0[q];
@@ -308,6 +313,7 @@ __device__ void tr_lane_body(const tr_params &prm, tr_job *jb, tr_ctx *ctxs)
         co->cost[0] = cs0;
         co->path[0] = pa0;
         co->next[0] = (uint8_t)nx0;
+        co->slot[0] = (uint8_t)sl0;
 #pragma unroll
         for (int w = 0; w < HW; w++) co->hist[0][w] = h0[w];
         if (n > 1) {
@@ -316,6 +322,7 @@ __device__ void tr_lane_body(const tr_params &prm, tr_job *jb, tr_ctx *ctxs)
             co->cost[1] = cs1;
             co->path[1] = pa1;
             co->next[1] = (uint8_t)nx1;
+            co->slot[1] = (uint8_t)sl1;
 #pragma unroll
             for (int w = 0; w < HW; w++) co->hist[1][w] = h1[w];
         }

@@ -34,6 +34,9 @@ typedef struct tr_ctx {
     double cost[TR_MAXK];
     uint32_t path[TR_MAXK];
     uint8_t next[TR_MAXK];
+    uint8_t slot[TR_MAXK];      /* slot of each survivor in the active bank: the
+                                   child index 2*parent_rank + branch of the
+                                   step that made it (the node act[] points at) */
     int8_t link[2][TR_SLOTS];
     uint32_t node_path[2][TR_SLOTS];
     double node_cost[2][TR_SLOTS];

@@ -6,6 +6,7 @@
  * It calls only the reference's public API (include/trellis.h:61-118,
  * include/ntf.h:23-36).
  */
+#include <stddef.h>
 #include <stdlib.h>
 #include <string.h>
 #include "trellis.h"
@@ -123,3 +124,90 @@ long ref_hash0_offset(const void *cv)
     if (!c->path_hash[0]) return -1;
     return (long)((const char *)c->path_hash[0] - (const char *)c);
 }
+
+/* Layout probe: sizes and field offsets of the context types, for checking
+ * a restatement of the layout. out: sizeof state/trellis/context, then the
+ * offsets of act, path_hash, hist_free, hist_fnum, trellis_mask, num_cands,
+ * idx, filter, prev_y, conv_fail, total_children, state_limit, hist, and of
+ * sdm_state_t's cost, path, next, hist, parent, path_list. */
+void ref_layout(long *out)
+{
+    int k = 0;
+    out[k++] = (long)sizeof(sdm_state_t);
+    out[k++] = (long)sizeof(sdm_trellis_t);
+    out[k++] = (long)sizeof(sdm_context_t);
+    out[k++] = (long)offsetof(sdm_trellis_t, act);
+    out[k++] = (long)offsetof(sdm_context_t, path_hash);
+    out[k++] = (long)offsetof(sdm_context_t, hist_free);
+    out[k++] = (long)offsetof(sdm_context_t, hist_fnum);
+    out[k++] = (long)offsetof(sdm_context_t, trellis_mask);
+    out[k++] = (long)offsetof(sdm_context_t, num_cands);
+    out[k++] = (long)offsetof(sdm_context_t, idx);
+    out[k++] = (long)offsetof(sdm_context_t, filter);
+    out[k++] = (long)offsetof(sdm_context_t, prev_y);
+    out[k++] = (long)offsetof(sdm_context_t, conv_fail);
+    out[k++] = (long)offsetof(sdm_context_t, total_children);
+    out[k++] = (long)offsetof(sdm_context_t, state_limit);
+    out[k++] = (long)offsetof(sdm_context_t, hist);
+    out[k++] = (long)offsetof(sdm_state_t, cost);
+    out[k++] = (long)offsetof(sdm_state_t, path);
+    out[k++] = (long)offsetof(sdm_state_t, next);
+    out[k++] = (long)offsetof(sdm_state_t, hist);
+    out[k++] = (long)offsetof(sdm_state_t, parent);
+    out[k++] = (long)offsetof(sdm_state_t, path_list);
+}
+
+/* What a context carries into its next steps, flattened:
+ *   meta[8]   num_cands, pos, pending, draining, idx & 1, trellis_num,
+ *             trellis_lat, hist_fnum
+ *   per survivor c < num_cands (act[c] of the active bank):
+ *             st[c*10 + k], cost[c], path[c], next[c], slot[c] (node index),
+ *             hist bits hist[c*256 + byte] (trellis_lat bits)
+ *   per node j < 64 of bank b (index b*64 + j): path, cost, and link =
+ *             -1 none, 0..63 node of the same bank, 1000 + 128*b' + j' node
+ *             j' of bank b' of `other`, -2 anything else. */
+void ref_export(const void *cv, const void *other, int *meta, double *st, double *cost, unsigned *path,
+                unsigned char *next, int *slot, unsigned char *hist, unsigned *npath, double *ncost, int *nlink)
+{
+    const sdm_context_t *c = (const sdm_context_t *)cv;
+    const sdm_context_t *o = (const sdm_context_t *)other;
+    const int bank = c->idx & 1;
+    meta[0] = (int)c->num_cands;
+    meta[1] = (int)c->pos;
+    meta[2] = (int)c->pending;
+    meta[3] = (int)c->draining;
+    meta[4] = bank;
+    meta[5] = (int)c->trellis_num;
+    meta[6] = (int)c->trellis_lat;
+    meta[7] = (int)c->hist_fnum;
+    for (unsigned i = 0; i < c->num_cands && i < SDM_TRELLIS_MAX_NUM; i++) {
+        const sdm_state_t *s = c->trellis[bank].act[i];
+        for (int k = 0; k < MAX_NTF_ORDER; k++) st[i * 10 + k] = s->state[k];
+        cost[i] = s->cost;
+        path[i] = s->path;
+        next[i] = s->next;
+        slot[i] = (int)(s - &c->trellis[bank].sdm[0]);
+        memcpy(hist + i * 256, c->hist[s->hist], (c->trellis_lat + 7) / 8);
+    }
+    for (int b = 0; b < 2; b++)
+        for (int j = 0; j < 64; j++) {
+            const sdm_state_t *s = &c->trellis[b].sdm[j];
+            const sdm_state_t *p = s->path_list;
+            int l = -2;
+            if (!p) l = -1;
+            else if (p >= &c->trellis[b].sdm[0] && p < &c->trellis[b].sdm[64]) l = (int)(p - &c->trellis[b].sdm[0]);
+            else if (o) {
+                for (int b2 = 0; b2 < 2; b2++)
+                    if (p >= &o->trellis[b2].sdm[0] && p < &o->trellis[b2].sdm[128])
+                        l = 1000 + 128 * b2 + (int)(p - &o->trellis[b2].sdm[0]);
+            }
+            npath[b * 64 + j] = s->path;
+            ncost[b * 64 + j] = s->cost;
+            nlink[b * 64 + j] = l;
+        }
+}
+
+/* Raw bytes of a context (size ref_ctx_size()), to load it into another
+ * implementation of the same layout, and back. */
+void ref_ctx_read(const void *cv, void *dst) { memcpy(dst, cv, sizeof(sdm_context_t)); }
+void ref_ctx_write(void *cv, const void *src) { memcpy(cv, src, sizeof(sdm_context_t)); }
