@@ -29,6 +29,7 @@
  * reference's operation order; build with -ffp-contract=off.
  */
 #include <emmintrin.h>
+#include <immintrin.h>
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
@@ -242,17 +243,100 @@ int fast_block_la(const double *a, const double *g, double *sv, const double *x,
     return bad;
 }
 
+/* The same on FMA hardware, with the chain from one index guess to the
+ * next shortened: the index is kept as a double (k = 0..levels-1 exactly),
+ * y_t = -1 + k step is formed from it as the reference forms it, and the
+ * next guess is w = (qn + h1) - (h1 step) k, truncated and clamped in
+ * doubles: one multiply-add, a round toward zero, a max and a min, where
+ * the integer form needs a table read, a subtraction and a conversion.
+ * The index as an integer (for the interval check) is off the chain. */
+/* trunc(w) clamped to [0, top] in doubles; NaN gives 0 */
+__attribute__((target("sse4.1"))) static inline double clampk(double top, double w)
+{
+    const __m128d z = _mm_setzero_pd();
+    const __m128d k = _mm_round_sd(z, _mm_set_sd(w), _MM_FROUND_TO_ZERO | _MM_FROUND_NO_EXC);
+    return _mm_cvtsd_f64(_mm_min_sd(_mm_max_sd(k, z), _mm_set_sd(top)));
+}
+
+/* m1 * m2 + ad, rounded once (used only for the index guess) */
+__attribute__((target("fma"))) static inline double fmad(double m1, double m2, double ad)
+{
+    return _mm_cvtsd_f64(_mm_fmadd_sd(_mm_set_sd(m1), _mm_set_sd(m2), _mm_set_sd(ad)));
+}
+
+template <int P>
+__attribute__((target("avx512f,fma,sse4.1"))) int fast_block_f(const double *a, const double *g, double *sv,
+                                                       const double *x, size_t n, const Quant &q)
+{
+    double s[P];
+    for (int i = 0; i < P; i++) s[i] = sv[i];
+    const double step = q.step, over = q.over, cx = q.cx, h1 = q.h1;
+    const double ch = q.c2 + h1, bh = h1 * step;
+    const double top = (double)(q.levels - 1);
+    const double c1 = q.c1, c2 = q.c2;
+    const double e0 = a[0] * c1, e1 = a[1] * c1, es = e0 + e1;
+    double kd;
+    {
+        double ns[P];
+        for (int i = 2; i < P - 1; i++) ns[i] = s[i] + s[i - 1] - g[i] * s[i + 1];
+        ns[P - 1] = s[P - 1] + s[P - 2];
+        double r = x[0] + a[2] * ns[2];
+        for (int i = 3; i < P; i++) r = r + a[i] * ns[i];
+        kd = clampk(top, (((r * c1 + c2) + e0 * (x[0] - g[0] * s[1])) + e1 * (s[1] - g[1] * s[2])) + es * s[0]);
+    }
+    int bad = 0;
+    for (size_t t = 0; t < n; t++) {
+        const double xv = x[t];
+        const double xn = (t + 1 < n) ? x[t + 1] : 0.0;
+        double ns[P], p[P];
+        ns[0] = s[0] - g[0] * s[1] + xv;
+        for (int i = 1; i < P - 1; i++) ns[i] = s[i] + s[i - 1] - g[i] * s[i + 1];
+        ns[P - 1] = s[P - 1] + s[P - 2];
+        for (int i = 0; i < P; i++) p[i] = a[i] * ns[i];
+        double ve = xv + p[0];
+        for (int i = 1; i < P; i++) ve = ve + p[i];
+        const int k = (int)kd;
+        const bool in = (ve >= q.lo[k]) & (ve < q.hi[k]);
+        const bool zero_ok = (k == 0) & ((ve >= over) | (ve != ve));
+        bad |= !(in | zero_ok);
+        /* next guess: the terms of s_{t-1} other than s0 (they arrive a
+         * step before s0) in four short multiply-add chains, which keeps the
+         * adders (the reference-order work needs them) free; s0's term last */
+        double c[4] = {fmad(cx, xn, ch), h1 * xv, 0.0, 0.0};
+        for (int i = 1; i < P; i++) {
+            const int j = (i - 1) & 3;
+            c[j] = (i <= 2 || i >= 5) ? fmad(q.r1[i], s[i], c[j]) : q.r1[i] * s[i];
+        }
+        const double qn = fmad(q.r1[0], s[0], (c[0] + c[1]) + (c[2] + c[3]));
+        const double kn = clampk(top, fmad(-bh, kd, qn));
+        const double y = -1.0 + kd * step;
+        ns[0] = ns[0] - y;
+        for (int i = 0; i < P; i++) s[i] = ns[i];
+        kd = kn;
+    }
+    for (int i = 0; i < P; i++) sv[i] = s[i];
+    return bad;
+}
+
+static bool have_fma()
+{
+    static const int ok = __builtin_cpu_supports("sse4.1") && __builtin_cpu_supports("fma") ? 1 : 0;
+    return ok != 0;
+}
+
 template <int P>
 void advance(const double *a, const double *g, double *s, const double *x, size_t n, const Quant &q,
              double lim, int64_t *redo)
 {
     const size_t BLK = 1024;
+    const bool vec = have_fma();
     for (size_t off = 0; off < n; off += BLK) {
         const size_t m = (n - off < BLK) ? n - off : BLK;
         if (q.ok) {
             double keep[P];
             for (int i = 0; i < P; i++) keep[i] = s[i];
             const int bad = (lim > 0.0) ? fast_block<P, true>(a, g, s, x + off, m, q, lim)
+                            : vec       ? fast_block_f<P>(a, g, s, x + off, m, q)
                                         : fast_block_la<P>(a, g, s, x + off, m, q);
             if (!bad) continue;
             for (int i = 0; i < P; i++) s[i] = keep[i];
