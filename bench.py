@@ -490,9 +490,10 @@ def main():
             r["estimator"] = {"host_chain_ms": round(est_ms, 3), "samples_all_channels": est_n,
                               "ns_per_sample_per_chain": round(est_ms * 1e6 / max(1, est_n // NCH), 3),
                               "reference_form_blocks": redo, "batches": nb,
-                              "note": "one host thread per channel; the segments whose estimates "
-                                      "have arrived launched as one batch whenever the previous "
-                                      "batch has finished"}
+                              "note": "two host threads per channel (index guess; its check against "
+                                      "the reference-order sum, which owns the exact state); the "
+                                      "segments whose estimates have arrived launched as one batch "
+                                      "whenever one of two batch streams is idle"}
         return r, n_out
 
     modes = ["seg0-first", "all-parallel"] if args.seed_mode == "both" else [args.seed_mode]

@@ -87,9 +87,9 @@ struct DasScratch {
     DevBuf<double> hin;                           /* host-buffer entry: staged input */
     DevBuf<unsigned long long> hbits;             /* ... packed output */
     DevBuf<float> hout;                           /* ... +-1.0 output */
-    /* seg0-first pipeline: the batch stream, batch completion events, the
+    /* seg0-first pipeline: two batch streams, batch completion events, the
      * estimate input staged in pinned host memory and its block events */
-    hipStream_t bs = 0;
+    hipStream_t bs[2] = {0, 0};
     std::vector<hipEvent_t> bev, blk;
     hipEvent_t ev_go = 0, ev_t0 = 0, ev_t1 = 0;
     double *h_est = nullptr;
@@ -98,7 +98,8 @@ struct DasScratch {
     int init_pipe()
     {
         if (pipe_ok) return 0;
-        if (hipStreamCreateWithFlags(&bs, hipStreamNonBlocking) != hipSuccess ||
+        if (hipStreamCreateWithFlags(&bs[0], hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&bs[1], hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&ev_go, hipEventDisableTiming) != hipSuccess ||
             hipEventCreate(&ev_t0) != hipSuccess || hipEventCreate(&ev_t1) != hipSuccess)
             return -1;
@@ -128,7 +129,8 @@ struct DasScratch {
     {
         for (hipEvent_t e : bev) hipEventDestroy(e);
         for (hipEvent_t e : blk) hipEventDestroy(e);
-        if (bs) hipStreamDestroy(bs);
+        for (hipStream_t x : bs)
+            if (x) hipStreamDestroy(x);
         if (ev_go) hipEventDestroy(ev_go);
         if (ev_t0) hipEventDestroy(ev_t0);
         if (ev_t1) hipEventDestroy(ev_t1);
@@ -569,37 +571,36 @@ struct EstChain {
     int64_t redo = 0;
 };
 
-/* advances e over [pos, end) of chain c, waiting for each staged block */
-static int est_feed(dsdhip_est_t *e, const EstChain *c, size_t pos, size_t end)
+/* the staged blocks of chain c covering [pos, end) have arrived */
+static int est_wait(void *cv, size_t pos, size_t end)
 {
-    while (pos < end) {
-        const size_t b = pos / EST_BLOCK;
-        const size_t bend = std::min((b + 1) * EST_BLOCK, end);
-        if (c->blk && hipEventSynchronize(c->blk[b]) != hipSuccess) return -1;
-        dsdhip_est_feed(e, c->hx + pos, bend - pos);
-        pos = bend;
-    }
+    const EstChain *c = (const EstChain *)cv;
+    if (!c->blk || end <= pos) return 0;
+    for (size_t b = pos / EST_BLOCK; b <= (end - 1) / EST_BLOCK; b++)
+        if (hipEventSynchronize(c->blk[b]) != hipSuccess) return -1;
     return 0;
 }
 
-static void est_worker(const dsdhip_das_t *d, std::vector<EstChain *> chains, std::atomic<int> *fail)
+static void est_publish(void *cv, int b, const double *st)
+{
+    EstChain *c = (EstChain *)cv;
+    for (int i = 0; i < 10; i++) c->est[(size_t)b * 10 + i] = st[i];
+    c->ready.store(b + 1, std::memory_order_release);
+}
+
+static void est_worker(const dsdhip_das_t *d, std::vector<EstChain *> chains, std::atomic<int> *fail,
+                       int threads)
 {
     for (EstChain *c : chains) {
         const int nb = (int)c->bnd.size();
         const auto t0 = std::chrono::steady_clock::now();
         dsdhip_est_t *e = dsdhip_est_create(d->a, d->g, d->order, d->state_limit, 16, c->init);
-        bool ok = e != nullptr;
-        size_t pos = 0;
-        for (int b = 0; b < nb; b++) {
-            const int64_t bd = c->bnd[(size_t)b];
-            if (ok && bd > (int64_t)pos && (size_t)bd <= c->count) {
-                ok = est_feed(e, c, pos, (size_t)bd) == 0;
-                pos = (size_t)bd;
-            }
-            if (ok) dsdhip_est_get_state(e, &c->est[(size_t)b * 10]);
-            c->ready.store(b + 1, std::memory_order_release);
+        bool ok = e != nullptr &&
+                  dsdhip_est_chain(e, c->hx, c->count, c->bnd.data(), nb, est_publish, est_wait, c, threads) == 0;
+        if (!ok) {
+            for (int b = c->ready.load(); b < nb; b++) c->ready.store(b + 1, std::memory_order_release);
+            fail->store(1);
         }
-        if (!ok) fail->store(1);
         if (e) {
             c->redo = dsdhip_est_redo_blocks(e);
             dsdhip_est_destroy(e);
@@ -676,7 +677,9 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
     const bool timing = d->timing != 0;
     if (hipEventRecord(sc.ev_go, d->stream) != hipSuccess) return -1;
     if (timing) hipEventRecord(sc.ev_t0, d->stream);
-    if (hipStreamWaitEvent(sc.bs, sc.ev_go, 0) != hipSuccess) return -1;
+    if (hipStreamWaitEvent(sc.bs[0], sc.ev_go, 0) != hipSuccess ||
+        hipStreamWaitEvent(sc.bs[1], sc.ev_go, 0) != hipSuccess)
+        return -1;
     /* stage the estimate input: every copy is queued now, block i of every
      * chain before block i + 1 of any, each followed by its event */
     {
@@ -707,23 +710,32 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
         }
     }
     int launched = 0, nbatch = 0;
-    auto launch = [&]() -> int {
+    int last[2] = {-1, -1};          /* last batch on each stream */
+    auto launch = [&](int si) -> int {
         const int n = (int)ent.size() - launched;
         if (n <= 0) return 0;
-        int rc = dsdhip_trellis_table_launch(tr, launched, n, sc.bs);
-        if (rc == 0 && hipEventRecord(sc.bev[(size_t)nbatch], sc.bs) != hipSuccess) rc = -1;
+        int rc = dsdhip_trellis_table_launch(tr, launched, n, sc.bs[si]);
+        if (rc == 0 && hipEventRecord(sc.bev[(size_t)nbatch], sc.bs[si]) != hipSuccess) rc = -1;
+        last[si] = nbatch;
         launched += n;
         nbatch++;
         return rc;
     };
-    auto idle = [&]() -> bool {
-        return nbatch == 0 || hipEventQuery(sc.bev[(size_t)nbatch - 1]) == hipSuccess;
+    auto idle = [&](int si) -> bool {
+        return last[si] < 0 || hipEventQuery(sc.bev[(size_t)last[si]]) == hipSuccess;
+    };
+    /* an idle stream, else (when nothing more will come) the one whose last
+     * batch started first; -1: wait */
+    auto pick = [&](bool final_batch) -> int {
+        if (idle(0)) return 0;
+        if (idle(1)) return 1;
+        return final_batch ? (last[0] < last[1] ? 0 : 1) : -1;
     };
     /* seg1: exact continuation, no estimate needed */
     int rc = 0;
     for (int t = 0; t < T && rc == 0; t++)
         for (int ch = 0; ch < nch && R[(size_t)t].S >= 2 && rc == 0; ch++) rc = fill(t, ch, 1, NULL);
-    if (rc == 0) rc = launch();
+    if (rc == 0) rc = launch(0);
     if (nchains > 0 && rc == 0) {
         /* estimator threads: chains dealt round-robin over at most 16
          * threads, one core left to this one */
@@ -733,7 +745,9 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
         for (size_t i = 0; i < nchains; i++) part[i % (size_t)nw].push_back(&chains[i]);
         std::atomic<int> fail{0};
         std::vector<std::thread> th;
-        for (int i = 0; i < nw; i++) th.emplace_back(est_worker, d, part[(size_t)i], &fail);
+        /* a chain's index guess gets a thread of its own where the cores allow */
+        const int per = (2 * (unsigned)nw + 1 <= hw) ? 2 : 1;
+        for (int i = 0; i < nw; i++) th.emplace_back(est_worker, d, part[(size_t)i], &fail, per);
         std::vector<int> taken(nchains, 0);
         for (;;) {
             bool all = true;
@@ -748,8 +762,9 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
             }
             if (rc != 0 || fail.load()) break;
             const int pending = (int)ent.size() - launched;
-            if (pending > 0 && (all || idle())) {
-                rc = launch();
+            const int si = pending > 0 ? pick(all) : -1;
+            if (si >= 0) {
+                rc = launch(si);
                 if (rc != 0) break;
             }
             if (all) break;
@@ -763,12 +778,12 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
             d->last_est_redo += chains[i].redo;
         }
     }
-    /* join the batch stream back into the DAS stream, then read results */
-    if (hipEventRecord(sc.bev[(size_t)nbatch], sc.bs) != hipSuccess ||
-        hipStreamWaitEvent(d->stream, sc.bev[(size_t)nbatch], 0) != hipSuccess)
-        rc = -1;
+    /* join the batch streams back into the DAS stream, then read results */
+    for (int si = 0; si < 2; si++)
+        if (last[si] >= 0 && hipStreamWaitEvent(d->stream, sc.bev[(size_t)last[si]], 0) != hipSuccess) rc = -1;
     if (rc != 0) {
-        hipStreamSynchronize(sc.bs);
+        hipStreamSynchronize(sc.bs[0]);
+        hipStreamSynchronize(sc.bs[1]);
         hipStreamSynchronize(d->stream);
         return -1;
     }

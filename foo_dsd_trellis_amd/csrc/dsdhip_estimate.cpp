@@ -33,6 +33,11 @@
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
+
+#include <atomic>
+#include <thread>
+#include <vector>
+
 #include "dsdhip_internal.h"
 
 namespace {
@@ -318,6 +323,87 @@ __attribute__((target("avx512f,fma,sse4.1"))) int fast_block_f(const double *a, 
     return bad;
 }
 
+/* The chain alone: the exact state advanced with guessed indices (no
+ * check), each index written to kb.  Exact wherever every guess is right,
+ * which a separate pass over the indices establishes (check_run). */
+template <int P>
+__attribute__((target("avx512f,fma,sse4.1"))) void guess_run(const double *a, const double *g, double *sv,
+                                                             const double *x, size_t n, const Quant &q,
+                                                             uint8_t *kb)
+{
+    double s[P];
+    for (int i = 0; i < P; i++) s[i] = sv[i];
+    const double step = q.step, cx = q.cx, h1 = q.h1;
+    const double ch = q.c2 + h1, bh = h1 * step;
+    const double top = (double)(q.levels - 1);
+    const double c1 = q.c1, c2 = q.c2;
+    const double e0 = a[0] * c1, e1 = a[1] * c1, es = e0 + e1;
+    double kd;
+    {
+        double ns[P];
+        for (int i = 2; i < P - 1; i++) ns[i] = s[i] + s[i - 1] - g[i] * s[i + 1];
+        ns[P - 1] = s[P - 1] + s[P - 2];
+        double r = x[0] + a[2] * ns[2];
+        for (int i = 3; i < P; i++) r = r + a[i] * ns[i];
+        kd = clampk(top, (((r * c1 + c2) + e0 * (x[0] - g[0] * s[1])) + e1 * (s[1] - g[1] * s[2])) + es * s[0]);
+    }
+    for (size_t t = 0; t < n; t++) {
+        const double xv = x[t];
+        const double xn = (t + 1 < n) ? x[t + 1] : 0.0;
+        double ns[P];
+        ns[0] = s[0] - g[0] * s[1] + xv;
+        for (int i = 1; i < P - 1; i++) ns[i] = s[i] + s[i - 1] - g[i] * s[i + 1];
+        ns[P - 1] = s[P - 1] + s[P - 2];
+        double c[4] = {fmad(cx, xn, ch), h1 * xv, 0.0, 0.0};
+        for (int i = 1; i < P; i++) {
+            const int j = (i - 1) & 3;
+            c[j] = (i <= 2 || i >= 5) ? fmad(q.r1[i], s[i], c[j]) : q.r1[i] * s[i];
+        }
+        const double qn = fmad(q.r1[0], s[0], (c[0] + c[1]) + (c[2] + c[3]));
+        const double kn = clampk(top, fmad(-bh, kd, qn));
+        kb[t] = (uint8_t)(int)kd;
+        const double y = -1.0 + kd * step;
+        ns[0] = ns[0] - y;
+        for (int i = 0; i < P; i++) s[i] = ns[i];
+        kd = kn;
+    }
+    for (int i = 0; i < P; i++) sv[i] = s[i];
+}
+
+/* The check of a guessed run: from the exact state sv, every index kb[t]
+ * against the reference-order v; returns n if all hold (sv then holds the
+ * exact state after the run), else the first sample whose index is wrong
+ * (sv holds the exact state before it). */
+template <int P>
+__attribute__((target("avx512f,fma,sse4.1"))) size_t check_run(const double *a, const double *g, double *sv,
+                                                               const double *x, size_t n, const Quant &q,
+                                                               const uint8_t *kb)
+{
+    double s[P];
+    for (int i = 0; i < P; i++) s[i] = sv[i];
+    const double step = q.step, over = q.over;
+    size_t t = 0;
+    for (; t < n; t++) {
+        const double xv = x[t];
+        double ns[P], p[P];
+        ns[0] = s[0] - g[0] * s[1] + xv;
+        for (int i = 1; i < P - 1; i++) ns[i] = s[i] + s[i - 1] - g[i] * s[i + 1];
+        ns[P - 1] = s[P - 1] + s[P - 2];
+        for (int i = 0; i < P; i++) p[i] = a[i] * ns[i];
+        double ve = xv + p[0];
+        for (int i = 1; i < P; i++) ve = ve + p[i];
+        const int k = kb[t];
+        const bool in = (ve >= q.lo[k]) & (ve < q.hi[k]);
+        const bool zero_ok = (k == 0) & ((ve >= over) | (ve != ve));
+        if (!(in | zero_ok)) break;
+        const double y = -1.0 + (double)k * step;
+        ns[0] = ns[0] - y;
+        for (int i = 0; i < P; i++) s[i] = ns[i];
+    }
+    for (int i = 0; i < P; i++) sv[i] = s[i];
+    return t;
+}
+
 static bool have_fma()
 {
     static const int ok = __builtin_cpu_supports("sse4.1") && __builtin_cpu_supports("fma") ? 1 : 0;
@@ -356,6 +442,143 @@ struct dsdhip_est {
     double s[10];
     int64_t redo;
 };
+
+/* ---- the chain on two threads ------------------------------------------
+ * The guessing loop alone (exact state, guessed indices) and the check of
+ * its indices (reference-order v against the index's interval, exact
+ * state) each cost less than the two together, so a chain runs them on
+ * two threads: the guesser writes indices into a ring, the checker (the
+ * calling thread) follows it and owns the exact state.  A wrong index
+ * stops the guesser; the checker takes that sample in the reference's
+ * form and the next block in the single-thread form, then starts the
+ * guesser again from its exact state.  Every published state is the
+ * checker's, so the results are those of the single-thread form. */
+#define EST_RING ((size_t)1 << 24)      /* indices in flight */
+#define EST_CHUNK ((size_t)1 << 16)     /* guesser step, divides EST_RING */
+
+struct EstSplit {
+    const dsdhip_est_t *proto;
+    const double *x;
+    size_t count;
+    int (*wait)(void *, size_t, size_t);
+    void *ctx;
+    std::vector<uint8_t> ring;
+    std::atomic<size_t> a_pos{0}, c_pos{0};
+    std::atomic<int> stop{0}, a_fail{0};
+};
+
+struct EstStart {
+    double s[10];
+};
+
+template <int P>
+static void guess_thread(EstSplit *sp, EstStart st, size_t pos)
+{
+    const dsdhip_est_t *e = sp->proto;
+    double s[P];
+    for (int i = 0; i < P; i++) s[i] = st.s[i];
+    while (pos < sp->count && !sp->stop.load(std::memory_order_relaxed)) {
+        const size_t end = std::min(sp->count, (pos / EST_CHUNK + 1) * EST_CHUNK);
+        while (end - sp->c_pos.load(std::memory_order_acquire) > EST_RING) {
+            if (sp->stop.load(std::memory_order_relaxed)) return;
+            std::this_thread::yield();
+        }
+        if (sp->wait && sp->wait(sp->ctx, pos, end) != 0) {
+            sp->a_fail.store(1);
+            return;
+        }
+        guess_run<P>(e->a, e->g, s, sp->x + pos, end - pos, e->q, &sp->ring[pos % EST_RING]);
+        pos = end;
+        sp->a_pos.store(pos, std::memory_order_release);
+    }
+}
+
+static bool split_usable(const dsdhip_est_t *e, size_t count)
+{
+    return e->q.ok && !(e->lim > 0.0) && have_fma() && __builtin_cpu_supports("avx512f") &&
+           count >= 4 * EST_CHUNK && std::thread::hardware_concurrency() > 1;
+}
+
+template <int P>
+static int chain_split(dsdhip_est_t *e, const double *x, size_t count, const int64_t *bnd, int nb,
+                       void (*publish)(void *, int, const double *), int (*wait)(void *, size_t, size_t),
+                       void *ctx)
+{
+    EstSplit sp;
+    sp.proto = e;
+    sp.x = x;
+    sp.count = count;
+    sp.wait = wait;
+    sp.ctx = ctx;
+    sp.ring.resize(EST_RING);
+    auto start = [&]() {
+        EstStart st;
+        memcpy(st.s, e->s, sizeof(st.s));
+        return st;
+    };
+    std::thread guesser(guess_thread<P>, &sp, start(), (size_t)0);
+    int fails = 0, rc = 0;
+    size_t pos = 0;
+    for (int b = 0; b < nb && rc == 0; b++) {
+        const int64_t bd = bnd[b];
+        if (bd > (int64_t)pos && (size_t)bd <= count) {
+            const size_t target = (size_t)bd;
+            while (pos < target && rc == 0) {
+                size_t avail;
+                while ((avail = sp.a_pos.load(std::memory_order_acquire)) <= pos) {
+                    if (sp.a_fail.load()) {
+                        rc = -1;
+                        break;
+                    }
+                    std::this_thread::yield();
+                }
+                if (rc) break;
+                size_t end = std::min(target, avail);
+                end = std::min(end, (pos / EST_RING + 1) * EST_RING);    /* no ring wrap inside a check */
+                const size_t ok = check_run<P>(e->a, e->g, e->s, x + pos, end - pos, e->q, &sp.ring[pos % EST_RING]);
+                if (ok == end - pos) {
+                    pos = end;
+                    sp.c_pos.store(pos, std::memory_order_release);
+                    continue;
+                }
+                /* a wrong index at pos + ok: the guesser's later indices are void */
+                sp.stop.store(1);
+                guesser.join();
+                pos += ok;
+                e->redo++;
+                ref_step<P>(e->a, e->g, e->s, x[pos], e->q, e->lim);
+                pos++;
+                const size_t blk = std::min(target - pos, (size_t)1024);
+                if (blk && wait && wait(ctx, pos, pos + blk) != 0) return -1;
+                if (blk) {
+                    advance<P>(e->a, e->g, e->s, x + pos, blk, e->q, e->lim, &e->redo);
+                    pos += blk;
+                }
+                sp.c_pos.store(pos, std::memory_order_release);
+                sp.a_pos.store(pos, std::memory_order_release);
+                if (++fails > 64) {
+                    /* guesses keep failing: finish in the single-thread form */
+                    for (int b2 = b; b2 < nb; b2++) {
+                        const int64_t bd2 = bnd[b2];
+                        if (bd2 > (int64_t)pos && (size_t)bd2 <= count) {
+                            if (wait && wait(ctx, pos, (size_t)bd2) != 0) return -1;
+                            advance<P>(e->a, e->g, e->s, x + pos, (size_t)bd2 - pos, e->q, e->lim, &e->redo);
+                            pos = (size_t)bd2;
+                        }
+                        publish(ctx, b2, e->s);
+                    }
+                    return 0;
+                }
+                sp.stop.store(0);
+                guesser = std::thread(guess_thread<P>, &sp, start(), pos);
+            }
+        }
+        if (rc == 0) publish(ctx, b, e->s);
+    }
+    sp.stop.store(1);
+    if (guesser.joinable()) guesser.join();
+    return rc;
+}
 
 extern "C" {
 
@@ -423,6 +646,38 @@ void dsdhip_est_feed(dsdhip_est_t *e, const double *x, size_t n)
     }
 }
 
+/* The estimate over x[0..count) with the state published at each boundary
+ * bnd[0..nb) (advanced only to boundaries past the current position and
+ * within count, src/threadpool.c:345-366).  wait(ctx, pos, end), if given,
+ * returns once x[pos..end) may be read (0) or on failure (nonzero).
+ * threads = 2 lets the index guess run on a thread of its own. */
+int dsdhip_est_chain(dsdhip_est_t *e, const double *x, size_t count, const int64_t *bnd, int nb,
+                     void (*publish)(void *, int, const double *), int (*wait)(void *, size_t, size_t),
+                     void *ctx, int threads)
+{
+    if (!e || !publish || (nb > 0 && !bnd)) return -1;
+    if (threads > 1 && split_usable(e, count)) {
+        switch (e->P) {
+        case 4: return chain_split<4>(e, x, count, bnd, nb, publish, wait, ctx);
+        case 5: return chain_split<5>(e, x, count, bnd, nb, publish, wait, ctx);
+        case 6: return chain_split<6>(e, x, count, bnd, nb, publish, wait, ctx);
+        case 7: return chain_split<7>(e, x, count, bnd, nb, publish, wait, ctx);
+        default: return chain_split<8>(e, x, count, bnd, nb, publish, wait, ctx);
+        }
+    }
+    size_t pos = 0;
+    for (int b = 0; b < nb; b++) {
+        const int64_t bd = bnd[b];
+        if (bd > (int64_t)pos && (size_t)bd <= count) {
+            if (wait && wait(ctx, pos, (size_t)bd) != 0) return -1;
+            dsdhip_est_feed(e, x + pos, (size_t)bd - pos);
+            pos = (size_t)bd;
+        }
+        publish(ctx, b, e->s);
+    }
+    return 0;
+}
+
 int64_t dsdhip_est_redo_blocks(const dsdhip_est_t *e) { return e->redo; }
 int dsdhip_est_fast(const dsdhip_est_t *e) { return e->q.ok; }
 
@@ -444,18 +699,19 @@ int dsdhip_estimate_boundaries_ex(const double *a, const double *g, int order, c
     dsdhip_est_t *e = dsdhip_est_create(a, g, order, state_limit, levels, init10);
     if (!e) return -1;
     e->q.c2 += index_bias;
-    size_t pos = 0;
-    for (int b = 0; b < nb; b++) {
-        const int64_t bd = bnd[b];
-        if (bd > (int64_t)pos && (size_t)bd <= count) {
-            dsdhip_est_feed(e, x + pos, (size_t)bd - pos);
-            pos = (size_t)bd;
-        }
-        dsdhip_est_get_state(e, out10 + (size_t)b * 10);
-    }
+    struct Out {
+        double *o;
+        int P;
+    } out = {out10, order};
+    const int rc = dsdhip_est_chain(e, x, count, bnd, nb,
+                                    [](void *c, int b, const double *st) {
+                                        Out *u = (Out *)c;
+                                        for (int i = 0; i < 10; i++) u->o[(size_t)b * 10 + i] = i < u->P ? st[i] : 0.0;
+                                    },
+                                    nullptr, &out, 2);
     if (redo_blocks) *redo_blocks = e->redo;
     dsdhip_est_destroy(e);
-    return 0;
+    return rc;
 }
 
 int dsdhip_estimate_boundaries(const double *a, const double *g, int order, const double *init10,

@@ -50,6 +50,14 @@ void dsdhip_est_set_state(dsdhip_est_t *e, const double *init10);
 void dsdhip_est_get_state(const dsdhip_est_t *e, double *out10);
 void dsdhip_est_feed(dsdhip_est_t *e, const double *x, size_t n);
 int64_t dsdhip_est_redo_blocks(const dsdhip_est_t *e);
+/* the estimate over x[0..count), the state at each boundary bnd[0..nb)
+ * handed to publish(ctx, b, state); wait(ctx, pos, end) (optional) returns
+ * 0 once x[pos..end) may be read.  With threads = 2 the index guess runs
+ * on a second thread where that applies (no limiter, fast table, FMA and
+ * AVX-512 hardware). */
+int dsdhip_est_chain(dsdhip_est_t *e, const double *x, size_t count, const int64_t *bnd, int nb,
+                     void (*publish)(void *, int, const double *), int (*wait)(void *, size_t, size_t),
+                     void *ctx, int threads);
 int dsdhip_est_fast(const dsdhip_est_t *e);
 
 #ifdef __cplusplus

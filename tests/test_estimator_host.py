@@ -148,3 +148,25 @@ def test_interval_table_is_the_reference_quantizer(oracle, levels):
             v = v * (1 + 1e-12)
         for v in vals:
             assert das.table_index(a0, levels, v) == ref_idx(v, a0, levels), (a0, levels, v)
+
+
+@pytest.mark.parametrize("bias", [0.0, 3e-5, -3e-5, 0.5])
+def test_two_thread_chain_matches_reference(oracle, bias):
+    """Long chains run the index guess on a second thread and check it on
+    the calling one; rare wrong guesses (a small bias) restart the guesser
+    from the checked state, frequent ones (a large bias) end in the
+    single-thread form.  Every boundary state stays the reference's."""
+    from foo_dsd_trellis_amd import das
+    rng = np.random.default_rng(21)
+    fi = oracle.ntf_auto(11289600)
+    n = (1 << 20) + 12345
+    x = _inputs(rng, n, "music")
+    x[700001] = np.nan                               # one non-finite input mid-chain
+    x[700002:700100] = 1e12
+    bnd = np.array([0, 5, 65536, 65537, 300000, 524287, 524288, 700000, 700050, 900001, n, n + 1], np.int64)
+    init = rng.standard_normal(10) * 0.01
+    got, redo = das.estimate_boundaries(fi, init, x, bnd, index_bias=bias)
+    want = chained(oracle, fi, init, x, bnd, 0.0, 16)
+    assert np.array_equal(_bits(got[:, :8]), _bits(want[:, :8]))
+    if bias != 0.0:
+        assert redo > 0
