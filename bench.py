@@ -214,7 +214,38 @@ def diff_loop_filter(steps, check_rows):
     return out
 
 
-def batched_uneven(chunks, cf, plain_tracks):
+def cpu_das_rate(pcm, fs_in, fs_out, seg_len, seg0_first, threads):
+    """The oracle's FP64 FIR chain (one thread per channel) and DAS (phases
+    over `threads` threads) on host PCM [nch][n]: output Msamples/s (all
+    channels) and the wall time."""
+    import ctypes as C
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    from foo_dsd_trellis_amd import fir
+    O = ol.oracle()
+    nch = pcm.shape[0]
+    fi = O.ntf_auto(fs_out)
+    O.lib.ora_das_set_threads.argtypes = [C.c_int]
+    O.lib.ora_das_set_threads(threads)
+    t0 = time.perf_counter()
+    ys = [None] * nch
+
+    def fir_ch(c):
+        ys[c] = ol.fir_chain(O, fs_in, fs_out, np.ascontiguousarray(pcm[c]), fir.PCM_FIR_GAIN)
+
+    th = [threading.Thread(target=fir_ch, args=(c,)) for c in range(nch)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    out, n, sp, plan = ol.das(O, fi, B, K, L, np.stack(ys), seg_len=seg_len, nseg=0,
+                              seg0_first=1 if seg0_first else 0, reposition=1)
+    wall = time.perf_counter() - t0
+    O.lib.ora_das_set_threads(1)
+    return nch * n / wall / 1e6, wall, len(plan)
+
+
+def batched_uneven(chunks, cf, plain_tracks, cpu_frames=0, cpu_threads=1):
     """The batched uneven workload, bounded: 16 tracks of 6-channel 48 kHz
     PCM (lengths drawn from U[5, 120] s, seeded band-limited Gaussian at
     -14 dBFS RMS) -> 9 half-band stages to DSD512/48 (24.576 MHz) -> DAS
@@ -243,7 +274,14 @@ def batched_uneven(chunks, cf, plain_tracks):
     dt = time.perf_counter() - t0
     out_bits = int((b1 - b0).sum()) * nch
     tune = plan.tune()
-    res = {"value": round(out_bits / dt / 1e6, 3), "unit": "Msamples/s",
+    rate = out_bits / dt
+    total = sum(lens) * R * nch                  # output samples of the whole workload
+    res = {"value": round(rate / 1e6, 3), "unit": "Msamples/s",
+           "timed_fraction": round(out_bits / total, 5),
+           "full_job_s_extrapolated": round(total / rate, 2),
+           "extrapolation": "whole workload (all tracks to their ends) at the timed chunks' rate; every "
+                            "timed chunk has all 16 tracks live, later chunks have fewer tracks and "
+                            "fewer jobs per launch, so this is the rate while the batch is full",
            "x_realtime_per_track": round((chunks - 1) * cf / fs_in / dt * len(pcm), 3),
            "timed_chunks": chunks - 1, "chunk_frames": cf, "tracks": len(pcm), "channels": nch,
            "track_seconds": [round(L_ / fs_in, 2) for L_ in lens], "tune": tune,
@@ -271,6 +309,15 @@ def batched_uneven(chunks, cf, plain_tracks):
         res["plain_per_track"] = {"value": round(tot / tt / 1e6, 3), "unit": "Msamples/s",
                                   "sample": "first chunk of %d tracks, one DAS launch per track "
                                             "(same segment length)" % plain_tracks}
+    if cpu_frames > 0:
+        x = pcm[0][:, :cpu_frames].cpu().numpy().astype(np.float32)
+        v, wall, nseg = cpu_das_rate(x, fs_in, fs_out, tune["seg_len"], False, cpu_threads)
+        cores, nproc, model = host_cpu()
+        res["cpu_baseline"] = {"value": round(v, 4), "unit": "Msamples/s", "cores": cpu_threads,
+                               "nproc": nproc, "cpu_model": model, "kind": "port",
+                               "sample": "first %d frames of track 0 (%d ch, %d segments per channel of %d): "
+                                         "oracle FP64 FIR chain + oracle DAS, all-parallel seeding as at "
+                                         "DSD512, %.1f s wall" % (cpu_frames, nch, nseg, tune["seg_len"], wall)}
     return res
 
 
@@ -422,6 +469,8 @@ def main():
                     help="chunks of the batched uneven workload (first untimed; 0: skip)")
     ap.add_argument("--batch-chunk-frames", type=int, default=12000)
     ap.add_argument("--batch-plain-tracks", type=int, default=2)
+    ap.add_argument("--batch-cpu-frames", type=int, default=4000,
+                    help="PCM frames of one track for the batched workload's CPU baseline (0: skip)")
     ap.add_argument("--stream-chunks", type=int, default=30,
                     help="timed chunks of the streaming room-correction workload (0: skip)")
     ap.add_argument("--stream-seg-len", type=int, default=8192)
@@ -569,7 +618,8 @@ def main():
         cpu = cpu_baseline(x, args.cpu_audio_seconds, modes[0] == "seg0-first", thr)
 
     lf = diff_loop_filter(args.lf_steps, args.lf_check_rows) if args.lf_steps > 0 else None
-    bu = (batched_uneven(args.batch_chunks, args.batch_chunk_frames, args.batch_plain_tracks)
+    bu = (batched_uneven(args.batch_chunks, args.batch_chunk_frames, args.batch_plain_tracks,
+                         args.batch_cpu_frames, args.cpu_threads if args.cpu_threads > 0 else host_cpu()[0])
           if args.batch_chunks > 1 else None)
     sr = streaming_room(args.stream_chunks, args.stream_seg_len) if args.stream_chunks > 0 else None
     nr = next_rows(bits0, n_out, args.next_seconds) if args.next_seconds > 0 else None
