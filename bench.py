@@ -542,7 +542,7 @@ def main():
                               "note": "two host threads per channel (index guess; its check against "
                                       "the reference-order sum, which owns the exact state); the "
                                       "segments whose estimates have arrived launched as one batch "
-                                      "whenever one of two batch streams is idle"}
+                                      "whenever the previous batch has finished (at least 1/32 of the jobs), the last batch on a second stream as soon as the estimate ends"}
         return r, n_out
 
     modes = ["seg0-first", "all-parallel"] if args.seed_mode == "both" else [args.seed_mode]

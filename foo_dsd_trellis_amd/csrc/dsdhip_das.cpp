@@ -87,9 +87,10 @@ struct DasScratch {
     DevBuf<double> hin;                           /* host-buffer entry: staged input */
     DevBuf<unsigned long long> hbits;             /* ... packed output */
     DevBuf<float> hout;                           /* ... +-1.0 output */
-    /* seg0-first pipeline: two batch streams, batch completion events, the
+    /* seg0-first pipeline: the batch streams, batch completion events, the
      * estimate input staged in pinned host memory and its block events */
-    hipStream_t bs[2] = {0, 0};
+    static const int NBS = 2;
+    hipStream_t bs[NBS] = {0, 0};
     std::vector<hipEvent_t> bev, blk;
     hipEvent_t ev_go = 0, ev_t0 = 0, ev_t1 = 0;
     double *h_est = nullptr;
@@ -98,9 +99,9 @@ struct DasScratch {
     int init_pipe()
     {
         if (pipe_ok) return 0;
-        if (hipStreamCreateWithFlags(&bs[0], hipStreamNonBlocking) != hipSuccess ||
-            hipStreamCreateWithFlags(&bs[1], hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&ev_go, hipEventDisableTiming) != hipSuccess ||
+        for (int i = 0; i < NBS; i++)
+            if (hipStreamCreateWithFlags(&bs[i], hipStreamNonBlocking) != hipSuccess) return -1;
+        if (hipEventCreateWithFlags(&ev_go, hipEventDisableTiming) != hipSuccess ||
             hipEventCreate(&ev_t0) != hipSuccess || hipEventCreate(&ev_t1) != hipSuccess)
             return -1;
         pipe_ok = 1;
@@ -615,8 +616,10 @@ static void est_worker(const dsdhip_das_t *d, std::vector<EstChain *> chains, st
  * from seg1's start runs on host threads (one chain per channel, reading
  * the input staged to pinned host memory block by block) and the segments
  * >= 2 are launched as their estimates arrive: a batch of everything ready
- * whenever the previous batch has finished (the launches run in order on
- * one stream), so the work left after the estimate ends is one batch.
+ * (seg1 with the first; at least a minimum size) whenever the previous one
+ * has finished, on one stream, and the last batch at once when the
+ * estimate ends, on a second stream, so the work left after the estimate
+ * is one batch.
  * Fills kept/slow of every segment >= 1.
  */
 static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned long long *segbits)
@@ -677,9 +680,8 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
     const bool timing = d->timing != 0;
     if (hipEventRecord(sc.ev_go, d->stream) != hipSuccess) return -1;
     if (timing) hipEventRecord(sc.ev_t0, d->stream);
-    if (hipStreamWaitEvent(sc.bs[0], sc.ev_go, 0) != hipSuccess ||
-        hipStreamWaitEvent(sc.bs[1], sc.ev_go, 0) != hipSuccess)
-        return -1;
+    for (int i = 0; i < DasScratch::NBS; i++)
+        if (hipStreamWaitEvent(sc.bs[i], sc.ev_go, 0) != hipSuccess) return -1;
     /* stage the estimate input: every copy is queued now, block i of every
      * chain before block i + 1 of any, each followed by its event */
     {
@@ -710,7 +712,9 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
         }
     }
     int launched = 0, nbatch = 0;
-    int last[2] = {-1, -1};          /* last batch on each stream */
+    const int NBS = DasScratch::NBS;
+    int last[DasScratch::NBS];       /* last batch on each stream */
+    for (int i = 0; i < NBS; i++) last[i] = -1;
     auto launch = [&](int si) -> int {
         const int n = (int)ent.size() - launched;
         if (n <= 0) return 0;
@@ -724,18 +728,22 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
     auto idle = [&](int si) -> bool {
         return last[si] < 0 || hipEventQuery(sc.bev[(size_t)last[si]]) == hipSuccess;
     };
-    /* an idle stream, else (when nothing more will come) the one whose last
-     * batch started first; -1: wait */
+    /* batches before the last run on stream 0, one after the other, each
+     * once it has a useful size (a batch holds its stream for about one
+     * segment's latency); stream 1 is kept for the last batch, which starts
+     * as soon as the estimate ends (streams that share a hardware queue
+     * would serialise, so there are only two) */
     auto pick = [&](bool final_batch) -> int {
-        if (idle(0)) return 0;
-        if (idle(1)) return 1;
-        return final_batch ? (last[0] < last[1] ? 0 : 1) : -1;
+        if (final_batch) return idle(0) ? 0 : 1;
+        return idle(0) ? 0 : -1;
     };
-    /* seg1: exact continuation, no estimate needed */
+    const int min_batch = std::max(32, total / 32);
+    /* seg1: exact continuation, no estimate needed; it goes with the first
+     * batch of estimated segments */
     int rc = 0;
     for (int t = 0; t < T && rc == 0; t++)
         for (int ch = 0; ch < nch && R[(size_t)t].S >= 2 && rc == 0; ch++) rc = fill(t, ch, 1, NULL);
-    if (rc == 0) rc = launch(0);
+    if (nchains == 0 && rc == 0) rc = launch(0);
     if (nchains > 0 && rc == 0) {
         /* estimator threads: chains dealt round-robin over at most 16
          * threads, one core left to this one */
@@ -762,7 +770,7 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
             }
             if (rc != 0 || fail.load()) break;
             const int pending = (int)ent.size() - launched;
-            const int si = pending > 0 ? pick(all) : -1;
+            const int si = (pending > 0 && (all || pending >= min_batch)) ? pick(all) : -1;
             if (si >= 0) {
                 rc = launch(si);
                 if (rc != 0) break;
@@ -779,11 +787,10 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
         }
     }
     /* join the batch streams back into the DAS stream, then read results */
-    for (int si = 0; si < 2; si++)
+    for (int si = 0; si < NBS; si++)
         if (last[si] >= 0 && hipStreamWaitEvent(d->stream, sc.bev[(size_t)last[si]], 0) != hipSuccess) rc = -1;
     if (rc != 0) {
-        hipStreamSynchronize(sc.bs[0]);
-        hipStreamSynchronize(sc.bs[1]);
+        for (int si = 0; si < NBS; si++) hipStreamSynchronize(sc.bs[si]);
         hipStreamSynchronize(d->stream);
         return -1;
     }
