@@ -154,6 +154,8 @@ struct dsdhip_das {
     double qfir[255];
     double *d_qfir;
     int est_device;              /* seg0-first estimate on one device lane per channel (timing) */
+    int batch_launch;            /* seg0-first: batches of ready segments instead of the host-fed launch */
+    double feed_limit_s;         /* host-fed launch: longest wait of one wave for its entry */
     /* last run */
     int last_S;
     std::vector<int> last_stitch;
@@ -238,6 +240,8 @@ static dsdhip_das_t *das_create(const double *a, const double *g, int order, int
     d->last_S = 0;
     d->last_slow = 0;
     d->est_device = 0;
+    d->batch_launch = 0;
+    d->feed_limit_s = 60.0;
     d->last_est_ms = 0.0;
     d->last_est_samples = 0;
     d->last_est_redo = 0;
@@ -614,12 +618,16 @@ static void est_worker(const dsdhip_das_t *d, std::vector<EstChain *> chains, st
  * seg0-first seeding after seg0 has run (src/dsp_plugin.c:2185-2266, 2288-
  * 2363): seg1 continues from the persistent context at once; the estimate
  * from seg1's start runs on host threads (one chain per channel, reading
- * the input staged to pinned host memory block by block) and the segments
- * >= 2 are launched as their estimates arrive: a batch of everything ready
- * (seg1 with the first; at least a minimum size) whenever the previous one
- * has finished, on one stream, and the last batch at once when the
- * estimate ends, on a second stream, so the work left after the estimate
- * is one batch.
+ * the input staged to pinned host memory block by block), and every
+ * segment >= 2 starts as soon as the estimate has passed its start: one
+ * host-fed persistent launch whose waves each take the next table entry
+ * and wait until this thread has published it (seg1 first, then the
+ * segments in the order their estimates arrive).  Where that launch does
+ * not apply (the one-lane kernel at K <= 2, the FMA filter), or when
+ * batch_launch is set, the entries are launched in batches instead: every
+ * entry ready when the previous batch has finished (at least a minimum
+ * size), on one stream, and the last batch at once when the estimate ends,
+ * on a second stream.
  * Fills kept/slow of every segment >= 1.
  */
 static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned long long *segbits)
@@ -676,14 +684,23 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
                                          r.in_cnt[(size_t)seg], r.keep[(size_t)seg],
                                          (uint64_t *)(segbits + r.woff[(size_t)(ch * r.S + seg)]), seed);
     };
-    /* the batch stream starts after seg0 and all earlier work */
+    /* the trellis streams start after seg0 and all earlier work */
     const bool timing = d->timing != 0;
     if (hipEventRecord(sc.ev_go, d->stream) != hipSuccess) return -1;
     if (timing) hipEventRecord(sc.ev_t0, d->stream);
     for (int i = 0; i < DasScratch::NBS; i++)
         if (hipStreamWaitEvent(sc.bs[i], sc.ev_go, 0) != hipSuccess) return -1;
+    /* host-fed launch: queued before the staging copies, so its waves are
+     * resident and waiting when the first entries are published */
+    int fed = 1;
+    if (!d->batch_launch) {
+        fed = dsdhip_trellis_feed_begin(tr, sc.bs[0], d->feed_limit_s);
+        if (fed < 0) return -1;
+    }
+    const bool feed = fed == 0;
     /* stage the estimate input: every copy is queued now, block i of every
      * chain before block i + 1 of any, each followed by its event */
+    int rc = 0;
     {
         size_t off = 0, ei = 0;
         std::vector<size_t> base(nchains, 0), first_ev(nchains, 0);
@@ -696,9 +713,9 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
             chains[i].hx = sc.h_est + base[i];
             chains[i].blk = &sc.blk[first_ev[i]];
         }
-        for (size_t b = 0;; b++) {
+        for (size_t b = 0; rc == 0; b++) {
             bool any = false;
-            for (size_t i = 0; i < nchains; i++) {
+            for (size_t i = 0; i < nchains && rc == 0; i++) {
                 EstChain &c = chains[i];
                 if (!c.blk || b * EST_BLOCK >= c.count) continue;
                 any = true;
@@ -706,44 +723,51 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
                 if (hipMemcpyAsync(sc.h_est + base[i] + b * EST_BLOCK, c.dx + b * EST_BLOCK, sizeof(double) * n,
                                    hipMemcpyDeviceToHost, d->stream) != hipSuccess ||
                     hipEventRecord(sc.blk[first_ev[i] + b], d->stream) != hipSuccess)
-                    return -1;
+                    rc = -1;
             }
             if (!any) break;
         }
     }
-    int launched = 0, nbatch = 0;
+    int launched = 0, nbatch = feed ? 1 : 0;
     const int NBS = DasScratch::NBS;
     int last[DasScratch::NBS];       /* last batch on each stream */
     for (int i = 0; i < NBS; i++) last[i] = -1;
+    if (feed) last[0] = 0;
     auto launch = [&](int si) -> int {
         const int n = (int)ent.size() - launched;
         if (n <= 0) return 0;
-        int rc = dsdhip_trellis_table_launch(tr, launched, n, sc.bs[si]);
-        if (rc == 0 && hipEventRecord(sc.bev[(size_t)nbatch], sc.bs[si]) != hipSuccess) rc = -1;
+        int rc2 = dsdhip_trellis_table_launch(tr, launched, n, sc.bs[si]);
+        if (rc2 == 0 && hipEventRecord(sc.bev[(size_t)nbatch], sc.bs[si]) != hipSuccess) rc2 = -1;
         last[si] = nbatch;
         launched += n;
         nbatch++;
-        return rc;
+        return rc2;
     };
     auto idle = [&](int si) -> bool {
         return last[si] < 0 || hipEventQuery(sc.bev[(size_t)last[si]]) == hipSuccess;
     };
-    /* batches before the last run on stream 0, one after the other, each
-     * once it has a useful size (a batch holds its stream for about one
-     * segment's latency); stream 1 is kept for the last batch, which starts
-     * as soon as the estimate ends (streams that share a hardware queue
-     * would serialise, so there are only two) */
+    /* batch form: batches before the last run on stream 0, one after the
+     * other, each once it has a useful size (a batch holds its stream for
+     * about one segment's latency); stream 1 is kept for the last batch,
+     * which starts as soon as the estimate ends (streams that share a
+     * hardware queue would serialise, so there are only two) */
     auto pick = [&](bool final_batch) -> int {
         if (final_batch) return idle(0) ? 0 : 1;
         return idle(0) ? 0 : -1;
     };
+    auto publish = [&]() -> int {
+        const int n = (int)ent.size();
+        if (dsdhip_trellis_feed_publish(tr, n) != 0) return -1;
+        launched = n;
+        return 0;
+    };
     const int min_batch = std::max(32, total / 32);
-    /* seg1: exact continuation, no estimate needed; it goes with the first
-     * batch of estimated segments */
-    int rc = 0;
+    /* seg1: exact continuation, no estimate needed; the host-fed launch
+     * starts it at once, the batch form with the first estimated batch */
     for (int t = 0; t < T && rc == 0; t++)
         for (int ch = 0; ch < nch && R[(size_t)t].S >= 2 && rc == 0; ch++) rc = fill(t, ch, 1, NULL);
-    if (nchains == 0 && rc == 0) rc = launch(0);
+    if (feed && rc == 0) rc = publish();
+    if (!feed && nchains == 0 && rc == 0) rc = launch(0);
     if (nchains > 0 && rc == 0) {
         /* estimator threads: chains dealt round-robin over at most 16
          * threads, one core left to this one */
@@ -770,13 +794,15 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
             }
             if (rc != 0 || fail.load()) break;
             const int pending = (int)ent.size() - launched;
-            const int si = (pending > 0 && (all || pending >= min_batch)) ? pick(all) : -1;
-            if (si >= 0) {
-                rc = launch(si);
-                if (rc != 0) break;
+            if (feed) {
+                if (pending > 0) rc = publish();
+            } else {
+                const int si = (pending > 0 && (all || pending >= min_batch)) ? pick(all) : -1;
+                if (si >= 0) rc = launch(si);
             }
+            if (rc != 0) break;
             if (all) break;
-            std::this_thread::sleep_for(std::chrono::microseconds(200));
+            std::this_thread::sleep_for(std::chrono::microseconds(feed ? 50 : 200));
         }
         for (auto &x : th) x.join();
         if (fail.load()) rc = -1;
@@ -786,12 +812,16 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
             d->last_est_redo += chains[i].redo;
         }
     }
-    /* join the batch streams back into the DAS stream, then read results */
+    /* a host-fed launch must not wait for entries that will never come */
+    if (feed && (rc != 0 || launched != total)) dsdhip_trellis_feed_abort(tr);
+    if (feed && hipEventRecord(sc.bev[0], sc.bs[0]) != hipSuccess) rc = -1;
+    /* join the trellis streams back into the DAS stream, then read results */
     for (int si = 0; si < NBS; si++)
         if (last[si] >= 0 && hipStreamWaitEvent(d->stream, sc.bev[(size_t)last[si]], 0) != hipSuccess) rc = -1;
     if (rc != 0) {
         for (int si = 0; si < NBS; si++) hipStreamSynchronize(sc.bs[si]);
         hipStreamSynchronize(d->stream);
+        if (feed) dsdhip_trellis_table_end(tr, d->stream, nullptr, nullptr, nullptr);
         return -1;
     }
     if (launched != total) return -1;
@@ -1240,6 +1270,26 @@ int dsdhip_das_estimator_device(dsdhip_das_t *d, int on)
 {
     if (!d) return -1;
     d->est_device = on ? 1 : 0;
+    return 0;
+}
+
+/* seg0-first launch form: 0 (default) one host-fed persistent launch whose
+ * waves start each segment as soon as its estimate is published; 1 batches
+ * of ready segments (the form used where the host-fed launch does not
+ * apply). */
+int dsdhip_das_batch_launch(dsdhip_das_t *d, int on)
+{
+    if (!d) return -1;
+    d->batch_launch = on ? 1 : 0;
+    return 0;
+}
+
+/* Longest wait of one wave of the host-fed launch for its entry, in
+ * seconds (default 60); a wave that waits longer leaves and the run fails. */
+int dsdhip_das_feed_limit(dsdhip_das_t *d, double seconds)
+{
+    if (!d || !(seconds >= 0.0) || seconds > 3600.0) return -1;
+    d->feed_limit_s = seconds;
     return 0;
 }
 

@@ -28,6 +28,11 @@ int dsdhip_trellis_table_fill(dsdhip_trellis_t *t, int j, int ctx, int copy_src,
 int dsdhip_trellis_table_launch(dsdhip_trellis_t *t, int first, int n, void *stream);
 int dsdhip_trellis_table_end(dsdhip_trellis_t *t, void *stream, int64_t *kept, int64_t *slow_steps,
                              int *err_out);
+/* host-fed launch over the table: one persistent launch whose waves wait for
+ * the entries the host publishes (0: running, 1: not applicable, -1 error) */
+int dsdhip_trellis_feed_begin(dsdhip_trellis_t *t, void *stream, double limit_s);
+int dsdhip_trellis_feed_publish(dsdhip_trellis_t *t, int n);
+int dsdhip_trellis_feed_abort(dsdhip_trellis_t *t);
 int dsdhip_trellis_timing_add(dsdhip_trellis_t *t, double ms, int64_t launches);
 int dsdhip_trellis_fill_job(const dsdhip_trellis_t *t, tr_job *jb, int ctx, int src, const double *d_in,
                             int64_t n_in, int64_t keep_from, uint64_t *d_out, const double *seed10);

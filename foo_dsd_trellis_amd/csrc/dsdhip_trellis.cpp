@@ -17,6 +17,10 @@
 
 extern "C" int tr_launch_jobs(const tr_params *prm, tr_job *d_jobs, int njobs,
                               tr_ctx *d_ctxs, hipStream_t stream);
+extern "C" int tr_feed_eligible(const tr_params *prm);
+extern "C" int tr_launch_feed(const tr_params *prm, const tr_job *h_jobs, tr_job *d_jobs, int njobs,
+                              const tr_feed_host *hc, tr_feed_dev *dc, tr_ctx *d_ctxs, uint64_t poll,
+                              uint64_t limit, hipStream_t stream);
 
 struct dsdhip_trellis {
     tr_params prm;
@@ -35,9 +39,13 @@ struct dsdhip_trellis {
     double *d_pin;           /* host-buffer entry: staged input / packed output, grow-only */
     uint64_t *d_pout;
     size_t pin_cap, pout_cap;
-    tr_job *table;               /* pipelined job table, pinned host memory (grow-only) */
+    tr_job *table;               /* pipelined job table, coherent pinned host memory (grow-only) */
     size_t table_cap, table_n;
     std::vector<int> table_src;  /* copy source of each entry */
+    /* host-fed launch over the table: control words and whether it runs */
+    tr_feed_host *h_feed;
+    tr_feed_dev *d_feed;
+    int feed_on;
 };
 
 static void ctx_fresh(tr_ctx *c)
@@ -79,6 +87,9 @@ dsdhip_trellis_t *dsdhip_trellis_create_coeffs(const double *a, const double *g,
     t->pin_cap = t->pout_cap = 0;
     t->table = NULL;
     t->table_cap = t->table_n = 0;
+    t->h_feed = NULL;
+    t->d_feed = NULL;
+    t->feed_on = 0;
     t->stream = 0;
     t->timing = 0;
     t->kernel_ms = 0.0;
@@ -161,6 +172,8 @@ void dsdhip_trellis_destroy(dsdhip_trellis_t *t)
     if (t->d_pin) hipFree(t->d_pin);
     if (t->d_pout) hipFree(t->d_pout);
     if (t->table) hipHostFree(t->table);
+    if (t->h_feed) hipHostFree(t->h_feed);
+    if (t->d_feed) hipFree(t->d_feed);
     delete t;
 }
 
@@ -366,11 +379,77 @@ int dsdhip_trellis_table_begin(dsdhip_trellis_t *t, int njobs)
         if (t->table) hipHostFree(t->table);
         t->table = NULL;
         t->table_cap = 0;
-        if (hipHostMalloc((void **)&t->table, sizeof(tr_job) * (size_t)njobs, 0) != hipSuccess) return -1;
+        /* coherent: a host-fed launch reads entries the host writes while it runs */
+        if (hipHostMalloc((void **)&t->table, sizeof(tr_job) * (size_t)njobs, hipHostMallocCoherent) !=
+            hipSuccess)
+            return -1;
         t->table_cap = (size_t)njobs;
     }
     t->table_n = (size_t)njobs;
     t->table_src.assign((size_t)njobs, -1);
+    t->feed_on = 0;
+    return 0;
+}
+
+/*
+ * Host-fed launch over the whole table (after table_begin): one launch on
+ * `stream` with one wave per entry, each waiting until the host has
+ * published its entry with dsdhip_trellis_feed_publish (entries 0..n-1
+ * complete, published in index order).  Every entry must be published, or the launch
+ * aborted with dsdhip_trellis_feed_abort, before the stream is joined; a
+ * wave that waits longer than limit_s seconds gives up (table_end then
+ * reports an error).  Returns 0, 1 when the host-fed form does not apply
+ * to these parameters (launch the entries with table_launch instead), or
+ * -1 on failure.
+ */
+int dsdhip_trellis_feed_begin(dsdhip_trellis_t *t, void *stream, double limit_s)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (!t) return -1;
+    if (!tr_feed_eligible(&t->prm)) return 1;
+    const int n = (int)t->table_n;
+    if (n == 0) return 1;
+    if (!t->h_feed &&
+        hipHostMalloc((void **)&t->h_feed, sizeof(tr_feed_host), hipHostMallocCoherent) != hipSuccess) {
+        t->h_feed = NULL;
+        return -1;
+    }
+    if (!t->d_feed && hipMalloc(&t->d_feed, sizeof(tr_feed_dev)) != hipSuccess) {
+        t->d_feed = NULL;
+        return -1;
+    }
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+        return -1;
+    tr_job *hd = NULL;
+    tr_feed_host *hc = NULL;
+    if (hipHostGetDevicePointer((void **)&hd, t->table, 0) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&hc, t->h_feed, 0) != hipSuccess)
+        return -1;
+    __atomic_store_n(&t->h_feed->ready, 0, __ATOMIC_RELEASE);
+    __atomic_store_n(&t->h_feed->abort, 0, __ATOMIC_RELEASE);
+    const uint64_t poll = (uint64_t)khz * 2 / 1000;           /* 2 us */
+    const uint64_t limit = (uint64_t)((double)khz * 1000.0 * limit_s);
+    if (tr_launch_feed(&t->prm, hd, t->d_jobs, n, hc, t->d_feed, t->d_ctx, poll ? poll : 1, limit, s) != 0)
+        return -1;
+    t->feed_on = 1;
+    return 0;
+}
+
+/* entries 0..n-1 of the table are complete (host-fed launch) */
+int dsdhip_trellis_feed_publish(dsdhip_trellis_t *t, int n)
+{
+    if (!t || !t->feed_on || n < 0 || (size_t)n > t->table_n) return -1;
+    __atomic_store_n(&t->h_feed->ready, n, __ATOMIC_RELEASE);
+    return 0;
+}
+
+/* the host-fed launch leaves without taking further entries */
+int dsdhip_trellis_feed_abort(dsdhip_trellis_t *t)
+{
+    if (!t || !t->feed_on) return -1;
+    __atomic_store_n(&t->h_feed->abort, 1, __ATOMIC_RELEASE);
     return 0;
 }
 
@@ -400,9 +479,22 @@ int dsdhip_trellis_table_end(dsdhip_trellis_t *t, void *stream, int64_t *kept, i
     hipStream_t s = (hipStream_t)stream;
     if (!t) return -1;
     const size_t n = t->table_n;
+    tr_feed_dev fd;
+    memset(&fd, 0, sizeof(fd));
+    if (t->feed_on &&
+        hipMemcpyAsync(&fd, t->d_feed, sizeof(fd), hipMemcpyDeviceToHost, s) != hipSuccess)
+        return -1;
     if (n && hipMemcpyAsync(t->table, t->d_jobs, sizeof(tr_job) * n, hipMemcpyDeviceToHost, s) != hipSuccess)
         return -1;
     if (hipStreamSynchronize(s) != hipSuccess) return -1;
+    if (t->feed_on) {
+        t->feed_on = 0;
+        /* a wave gave up waiting, or the launch was aborted: entries may not have run */
+        if (fd.err || fd.abort || fd.done != (int)n) {
+            if (err_out) *err_out = 3;
+            return -1;
+        }
+    }
     int err = 0;
     for (size_t j = 0; j < n; j++) {
         if (kept) kept[j] = t->table[j].kept;

@@ -22,6 +22,7 @@
  * History is carried by value per survivor (HW 64-bit words of L bits).
  */
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include "tr_types.h"
 
 typedef unsigned long long u64;
@@ -243,6 +244,12 @@ __device__ __forceinline__ u64 bp_u64(u64 v, int from)
 }
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ long long uni_ll(long long v)
+{
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(u64)v);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)((u64)v >> 32));
+    return (long long)(((u64)hi << 32) | lo);
+}
 __device__ __forceinline__ long long rl_ll(long long v, int l)
 {
     const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u64)v, l);
@@ -384,12 +391,17 @@ __device__ int wide_prune(TrLds<P, HW> &s, int nb, int m, int K, const tr_ctx *s
 /* One job on one wave; s is the wave's own LDS region.  Nothing here
  * synchronises beyond the wave, so several waves of a workgroup may run
  * different jobs (and different numbers of them). */
-template <int P, int HW, bool F8 = false>
+template <int P, int HW, bool F8 = false, bool UJ = false>
 __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr_ctx *ctxs,
                                             TrLds<P, HW> &s, const int lane)
 {
-    const tr_ctx *c0 = &ctxs[jb->ctx_in];
-    const tr_ctx *snap = jb->snap >= 0 ? &ctxs[jb->snap] : nullptr;
+    /* UJ: the descriptor sits in LDS, whose loads the compiler does not
+     * treat as wave-uniform; its fields are then made uniform explicitly */
+    auto fi = [](int v) { return UJ ? uni(v) : v; };
+    auto fl = [](long long v) { return UJ ? uni_ll(v) : v; };
+    const int ctx_in = fi(jb->ctx_in), snap_i = fi(jb->snap);
+    const tr_ctx *c0 = &ctxs[ctx_in];
+    const tr_ctx *snap = snap_i >= 0 ? &ctxs[snap_i] : nullptr;
     const int K = prm.K, L = prm.L;
     const uint32_t mask = prm.mask;
     const bool groups_ok = (prm.B <= 8) && !prm.force_slow;
@@ -409,9 +421,10 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
     for (int w = 0; w < HW; w++) rhist[w] = 0;
     {
         if (lane < num_cands) {
-            const bool sd = jb->seed != 0;
+            const bool sd = fi(jb->seed) != 0;
 #pragma unroll
-            for (int q = 0; q < P; q++) rst[q] = sd ? jb->seed_st[q] : c0->st[lane][q];
+            for (int q = 0; q < P; q++)
+                rst[q] = sd ? __longlong_as_double(fl(bits(jb->seed_st[q]))) : c0->st[lane][q];
 #pragma unroll
             for (int w = 0; w < HW; w++) rhist[w] = c0->hist[lane][w];
             rcost = sd ? 0.0 : c0->cost[lane];
@@ -422,7 +435,7 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
         if (lane < TR_MAXK) s.src[lane] = (int8_t)lane;
         for (int b = 0; b < 2; b++) {
             int8_t l = c0->link[b][lane];
-            if (jb->copy) {
+            if (fi(jb->copy)) {
                 if (l >= TR_LINK_FOREIGN) err = 2;
                 else if (l >= 0) l = (int8_t)(l + TR_LINK_FOREIGN);
             }
@@ -438,9 +451,9 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
     const u64 lowerm = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     /* global address space: these accesses count against vmcnt only, so the
      * prefetched block stays in flight across the step's LDS waits */
-    const GPTR(double) in = (const GPTR(double))jb->in;
-    GPTR(u64) out = (GPTR(u64))jb->out;
-    const int64_t n_in = jb->n_in, keep_from = jb->keep_from;
+    const GPTR(double) in = (const GPTR(double))fl((long long)jb->in);
+    GPTR(u64) out = (GPTR(u64))fl((long long)jb->out);
+    const int64_t n_in = fl(jb->n_in), keep_from = fl(jb->keep_from);
     u64 acc = 0;
     int nacc = 0;
     int64_t kept = 0, slow_steps = 0;
@@ -739,8 +752,9 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
     }
 
     /* ---- store the final context ---- */
-    if (jb->ctx_out >= 0) {
-        tr_ctx *co = &ctxs[jb->ctx_out];
+    const int ctx_out = fi(jb->ctx_out);
+    if (ctx_out >= 0) {
+        tr_ctx *co = &ctxs[ctx_out];
         if (myr < num_cands) {
 #pragma unroll
             for (int q = 0; q < P; q++) co->st[myr][q] = rst[q];
@@ -827,6 +841,105 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(TrWave
     }
 }
 
+/* Host-fed queue: the wait of one wave (lane 0) for job j.  Returns 0 once
+ * the host has published j, 1 on the host's abort flag or when the wait
+ * exceeds `limit` wall-clock ticks (dc->err is then set).  Waves read the
+ * device copy of the published count; the host word is read by one wave at
+ * a time, at most once per `poll` ticks (a compare-and-swap on the time of
+ * the last read), which raises the device copy for every waiting wave. */
+__device__ int feed_wait(const tr_feed_host *hc, tr_feed_dev *dc, int j, uint64_t poll, uint64_t limit)
+{
+    const uint64_t t_start = (uint64_t)wall_clock64();
+    for (;;) {
+        const int r = __hip_atomic_load(&dc->ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if (r > j &&
+            __hip_atomic_load(&hc->ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) > j)
+            return 0;
+        if (__hip_atomic_load(&dc->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return 1;
+        const uint64_t now = (uint64_t)wall_clock64();
+        uint64_t last = __hip_atomic_load(&dc->last_poll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (now > last && now - last >= poll &&
+            __hip_atomic_compare_exchange_strong(&dc->last_poll, &last, now, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            const int h = __hip_atomic_load(&hc->ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+            const int a = __hip_atomic_load(&hc->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_fetch_max(&dc->ready, h, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (a) {
+                __hip_atomic_store(&dc->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return 1;
+            }
+            if (h > j) return 0;
+            continue;
+        }
+        if (now > t_start && now - t_start > limit) {
+            __hip_atomic_fetch_or(&dc->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return 1;
+        }
+        __builtin_amdgcn_s_sleep(16);
+    }
+}
+
+static_assert(sizeof(tr_job) % 8 == 0, "tr_job is copied as 64-bit words");
+static_assert(offsetof(tr_job, slow_steps) == offsetof(tr_job, kept) + 8 &&
+                  offsetof(tr_job, err) == offsetof(tr_job, kept) + 16 && sizeof(tr_job) == offsetof(tr_job, kept) + 24,
+              "the results are the descriptor's last three words");
+
+/* Host-fed launch, one wave per workgroup and one job per workgroup:
+ * workgroup j waits until the host has published descriptor j (in coherent
+ * pinned host memory), copies it into LDS, runs it, and stores the
+ * descriptor with its results into the device table `jobs` (read back by
+ * the host after the launch).  The host publishes in index order and
+ * workgroups are dispatched in index order, so the waiting workgroups are
+ * the next ones to run; a wave leaves early on the host's abort flag or
+ * after a wait longer than `limit`, so the grid always drains. */
+template <int P, int HW>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TrWaves<HW>::value, 8))) void tr_run_feed(
+    tr_params prm, const tr_job *hjobs, tr_job *jobs, const tr_feed_host *hc, tr_feed_dev *dc, tr_ctx *ctxs,
+    uint64_t poll, uint64_t limit)
+{
+    __shared__ TrLds<P, HW> s;
+    __shared__ tr_job jd;
+    constexpr int NW = (int)(sizeof(tr_job) / 8);
+    const int lane = threadIdx.x;
+    const int j = blockIdx.x;
+    int st = 0;
+    if (lane == 0) st = feed_wait(hc, dc, j, poll, limit);
+    if (__builtin_amdgcn_readfirstlane(st)) return;
+    /* lane w copies word w of the descriptor into LDS */
+    if (lane < NW)
+        reinterpret_cast<uint64_t *>(&jd)[lane] = __hip_atomic_load(
+            reinterpret_cast<const uint64_t *>(&hjobs[j]) + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    wsync();
+    if (!uni(jd.skip)) tr_job_body<P, HW, false, true>(prm, &jd, ctxs, s, lane);
+    wsync();
+    if (lane < NW) reinterpret_cast<uint64_t *>(&jobs[j])[lane] = reinterpret_cast<const uint64_t *>(&jd)[lane];
+    if (lane == 0) __hip_atomic_fetch_add(&dc->done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* resets the device side of a host-fed queue (a kernel, on the launch's stream) */
+__global__ void tr_feed_init(tr_feed_dev *dc)
+{
+    if (threadIdx.x == 0) {
+        dc->done = 0;
+        dc->ready = 0;
+        dc->abort = 0;
+        dc->err = 0;
+        dc->last_poll = 0;
+        dc->t0 = 0;
+    }
+}
+
+#define TF_INST(P) \
+    template __global__ void tr_run_feed<P, 1>(tr_params, const tr_job *, tr_job *, const tr_feed_host *, \
+                                              tr_feed_dev *, tr_ctx *, uint64_t, uint64_t); \
+    template __global__ void tr_run_feed<P, 2>(tr_params, const tr_job *, tr_job *, const tr_feed_host *, \
+                                              tr_feed_dev *, tr_ctx *, uint64_t, uint64_t);
+TF_INST(4)
+TF_INST(5)
+TF_INST(6)
+TF_INST(7)
+TF_INST(8)
+
 #define TR_INST(P, HW) \
     template __global__ void tr_run_jobs<P, HW>(tr_params, tr_job *, tr_ctx *); \
     template __global__ void tr_run_queue<P, HW, 1>(tr_params, tr_job *, int, int *, tr_ctx *); \
@@ -904,6 +1017,38 @@ extern "C" int tr_launch_queue(const tr_params *prm, tr_job *d_jobs, int njobs, 
 #define TQ_CASE_P(PP) TQ_CASE_W(PP, 1) TQ_CASE_W(PP, 2) TQ_CASE_W(PP, 4) TQ_CASE_W(PP, 8) \
     TQ_CASE_W(PP, 16) TQ_CASE_W(PP, 32)
     TQ_CASE_P(4) TQ_CASE_P(5) TQ_CASE_P(6) TQ_CASE_P(7) TQ_CASE_P(8)
+    return -1;
+}
+
+/* Whether the host-fed launch covers prm: the one-wave kernel without the
+ * FMA filter, histories up to 128 samples. */
+extern "C" int tr_feed_eligible(const tr_params *prm)
+{
+    return !tr_lane_eligible(prm) && !(prm->fma_o8 && prm->P == 8) && prm->P >= 4 && prm->P <= 8 &&
+           prm->L >= 1 && prm->L <= 128;
+}
+
+/* Host-fed launch: resets dc on `stream`, then one one-wave workgroup per
+ * entry of h_jobs (coherent pinned host memory, visible to the device at
+ * that address) runs it once the host has published it through hc.
+ * poll / limit in wall-clock ticks. */
+extern "C" int tr_launch_feed(const tr_params *prm, const tr_job *h_jobs, tr_job *d_jobs, int njobs,
+                              const tr_feed_host *hc, tr_feed_dev *dc, tr_ctx *d_ctxs, uint64_t poll,
+                              uint64_t limit, hipStream_t stream)
+{
+    if (njobs <= 0) return 0;
+    if (!tr_feed_eligible(prm)) return -1;
+    const int P = prm->P, HW = prm->L > 64 ? 2 : 1;
+    hipLaunchKernelGGL(tr_feed_init, dim3(1), dim3(64), 0, stream, dc);
+    if (hipGetLastError() != hipSuccess) return -1;
+    dim3 grid((unsigned)njobs), block(64);
+#define TFL_CASE(PP, HH) \
+    if (P == PP && HW == HH) { \
+        hipLaunchKernelGGL((tr_run_feed<PP, HH>), grid, block, 0, stream, *prm, h_jobs, d_jobs, hc, dc, d_ctxs, \
+                           poll, limit); \
+        return hipGetLastError() == hipSuccess ? 0 : -1; }
+    TFL_CASE(4, 1) TFL_CASE(5, 1) TFL_CASE(6, 1) TFL_CASE(7, 1) TFL_CASE(8, 1)
+    TFL_CASE(4, 2) TFL_CASE(5, 2) TFL_CASE(6, 2) TFL_CASE(7, 2) TFL_CASE(8, 2)
     return -1;
 }
 

@@ -75,4 +75,23 @@ typedef struct tr_job {
     int32_t pad2;
 } tr_job;
 
+/* Host-fed launch (tr_run_feed): the host fills job descriptors in pinned,
+ * coherent host memory while the launch runs and publishes how many are
+ * complete; the wave of job j waits until the host has published it.  Host
+ * side (coherent pinned memory, written by the host only): */
+typedef struct tr_feed_host {
+    int32_t ready;         /* descriptors 0..ready-1 are complete */
+    int32_t abort;         /* nonzero: leave without taking further jobs */
+} tr_feed_host;
+
+/* Device side (device memory, reset by tr_feed_init before each launch): */
+typedef struct tr_feed_dev {
+    int32_t done;          /* jobs run to the end */
+    int32_t ready;         /* last published count seen by any wave */
+    int32_t abort;         /* the host's abort flag, seen by a wave */
+    int32_t err;           /* 1: a wave gave up waiting (time limit) */
+    uint64_t last_poll;    /* wall clock of the last read of the host word */
+    uint64_t t0;           /* wall clock when the first wave started */
+} tr_feed_dev;
+
 #endif

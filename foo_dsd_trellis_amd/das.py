@@ -134,6 +134,16 @@ class Das:
         instead of host threads (timing only; same bits)."""
         check(lib().dsdhip_das_estimator_device(self.h, 1 if on else 0), "estimator_device")
 
+    def batch_launch(self, on=True):
+        """seg0-first: launch the estimated segments in batches instead of
+        the host-fed persistent launch (same bits)."""
+        check(lib().dsdhip_das_batch_launch(self.h, 1 if on else 0), "batch_launch")
+
+    def feed_limit(self, seconds):
+        """Longest wait of one wave of the host-fed launch for its segment;
+        a wave that waits longer leaves and the run fails."""
+        check(lib().dsdhip_das_feed_limit(self.h, float(seconds)), "feed_limit")
+
     def last_estimate(self):
         """(longest estimator chain ms, samples estimated, redo blocks, batches)
         of the last seg0-first run."""

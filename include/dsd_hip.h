@@ -331,6 +331,17 @@ int dsdhip_das_last_info(const dsdhip_das_t *d, int *nseg, int64_t *slow_steps, 
  * per channel before one launch instead (same bits; for timing).  No
  * reference counterpart. */
 int dsdhip_das_estimator_device(dsdhip_das_t *d, int on);
+/* seg0-first launch form: on = 0 (default) one persistent launch whose waves
+ * each take the next segment and wait (bounded) until the host has
+ * published its estimate, so every segment starts as soon as its seed
+ * state exists; on = 1 batches of ready segments, each batch launched when
+ * the previous one has finished (the form used at K <= 2 and with the FMA
+ * filter).  Same bits.  No reference counterpart. */
+int dsdhip_das_batch_launch(dsdhip_das_t *d, int on);
+/* Longest wait, in seconds (default 60), of one wave of the host-fed launch
+ * for its segment; a wave that waits longer leaves and the run returns -1.
+ * No reference counterpart. */
+int dsdhip_das_feed_limit(dsdhip_das_t *d, double seconds);
 /* Estimator figures of the last seg0-first run: longest chain's busy time,
  * samples estimated over all channels, blocks the fast form handed back to
  * the reference form, pipelined trellis launches. */

@@ -153,3 +153,43 @@ def test_stream_dsd256_seg0_first_count(gpu, oracle):
         F = (f1 - f0) * S.R
         assert n == want.shape[1] == (F - 32 if k == 0 else F), (k, n)
         assert np.array_equal(S.bits_numpy(n), want), k
+
+
+def test_host_fed_launch_equals_batches(gpu, oracle):
+    """The host-fed persistent launch (every segment starts once its estimate
+    is published) and the batch form give the oracle's bits; six channels,
+    so the estimate chains publish interleaved."""
+    nch, F, seg = 6, 180000, 16384
+    x = _pink(nch, F, 91)
+    fi = oracle.ntf_auto(RATE)
+    print("case: host-fed vs batches, 6 ch, F=%d seg=%d K=16 L=32" % (F, seg), flush=True)
+    ref, n, _, _ = ol.das(oracle, fi, B, 16, 32, x, seg_len=seg, nseg=0, seg0_first=1, reposition=1)
+    outs = []
+    for batches in (False, True):
+        d = D.Das(fi, B, 16, 32, nch, seg_len=seg, seg0_first=True, reposition=True)
+        d.batch_launch(batches)
+        y, m = d.run(x)
+        ms, ns, redo, nb = d.last_estimate()
+        assert m == n
+        assert (nb == 1) if not batches else (nb >= 1)
+        assert np.array_equal(y[:, :n], ref[:, :n]), "batches=%s" % batches
+        outs.append(y)
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_host_fed_wait_limit_ends_the_launch(gpu, oracle):
+    """A wave that waits longer than the limit for its segment leaves: with
+    a zero limit the run fails with an error instead of waiting (the
+    condition is forced from the host; every wave exits), and a new object
+    with the default limit then runs bit-exact."""
+    nch, F, seg = 2, 120000, 16384
+    x = _pink(nch, F, 93)
+    fi = oracle.ntf_auto(RATE)
+    d = D.Das(fi, B, 16, 32, nch, seg_len=seg, seg0_first=True, reposition=True)
+    d.feed_limit(0.0)
+    with pytest.raises(Exception):
+        d.run(x)
+    ref, n, _, _ = ol.das(oracle, fi, B, 16, 32, x, seg_len=seg, nseg=0, seg0_first=1, reposition=1)
+    d2 = D.Das(fi, B, 16, 32, nch, seg_len=seg, seg0_first=True, reposition=True)
+    y, m = d2.run(x)
+    assert m == n and np.array_equal(y[:, :n], ref[:, :n])
