@@ -241,7 +241,7 @@ static dsdhip_das_t *das_create(const double *a, const double *g, int order, int
     d->last_slow = 0;
     d->est_device = 0;
     d->batch_launch = 0;
-    d->feed_limit_s = 60.0;
+    d->feed_limit_s = 20.0;
     d->last_est_ms = 0.0;
     d->last_est_samples = 0;
     d->last_est_redo = 0;
@@ -690,14 +690,6 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
     if (timing) hipEventRecord(sc.ev_t0, d->stream);
     for (int i = 0; i < DasScratch::NBS; i++)
         if (hipStreamWaitEvent(sc.bs[i], sc.ev_go, 0) != hipSuccess) return -1;
-    /* host-fed launch: queued before the staging copies, so its waves are
-     * resident and waiting when the first entries are published */
-    int fed = 1;
-    if (!d->batch_launch) {
-        fed = dsdhip_trellis_feed_begin(tr, sc.bs[0], d->feed_limit_s);
-        if (fed < 0) return -1;
-    }
-    const bool feed = fed == 0;
     /* stage the estimate input: every copy is queued now, block i of every
      * chain before block i + 1 of any, each followed by its event */
     int rc = 0;
@@ -728,11 +720,20 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
             if (!any) break;
         }
     }
+    /* host-fed launch: on the DAS stream behind every staging copy, since
+     * its waves wait for estimates that need those copies (a launch queued
+     * ahead of them, on any stream sharing their hardware queue, would
+     * never see its entries published) */
+    int fed = 1;
+    if (rc == 0 && !d->batch_launch) {
+        fed = dsdhip_trellis_feed_begin(tr, d->stream, d->feed_limit_s);
+        if (fed < 0) rc = -1;
+    }
+    const bool feed = fed == 0;
     int launched = 0, nbatch = feed ? 1 : 0;
     const int NBS = DasScratch::NBS;
     int last[DasScratch::NBS];       /* last batch on each stream */
     for (int i = 0; i < NBS; i++) last[i] = -1;
-    if (feed) last[0] = 0;
     auto launch = [&](int si) -> int {
         const int n = (int)ent.size() - launched;
         if (n <= 0) return 0;
@@ -812,10 +813,10 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
             d->last_est_redo += chains[i].redo;
         }
     }
-    /* a host-fed launch must not wait for entries that will never come */
+    /* a host-fed launch must not wait for entries that will never come; it
+     * runs on the DAS stream itself, so it needs no join */
     if (feed && (rc != 0 || launched != total)) dsdhip_trellis_feed_abort(tr);
-    if (feed && hipEventRecord(sc.bev[0], sc.bs[0]) != hipSuccess) rc = -1;
-    /* join the trellis streams back into the DAS stream, then read results */
+    /* join the batch streams back into the DAS stream, then read results */
     for (int si = 0; si < NBS; si++)
         if (last[si] >= 0 && hipStreamWaitEvent(d->stream, sc.bev[(size_t)last[si]], 0) != hipSuccess) rc = -1;
     if (rc != 0) {
@@ -1285,7 +1286,7 @@ int dsdhip_das_batch_launch(dsdhip_das_t *d, int on)
 }
 
 /* Longest wait of one wave of the host-fed launch for its entry, in
- * seconds (default 60); a wave that waits longer leaves and the run fails. */
+ * seconds (default 20); a wave that waits longer leaves and the run fails. */
 int dsdhip_das_feed_limit(dsdhip_das_t *d, double seconds)
 {
     if (!d || !(seconds >= 0.0) || seconds > 3600.0) return -1;

@@ -20,7 +20,7 @@ extern "C" int tr_launch_jobs(const tr_params *prm, tr_job *d_jobs, int njobs,
 extern "C" int tr_feed_eligible(const tr_params *prm);
 extern "C" int tr_launch_feed(const tr_params *prm, const tr_job *h_jobs, tr_job *d_jobs, int njobs,
                               const tr_feed_host *hc, tr_feed_dev *dc, tr_ctx *d_ctxs, uint64_t poll,
-                              uint64_t limit, hipStream_t stream);
+                              uint64_t limit, uint64_t max_iter, hipStream_t stream);
 
 struct dsdhip_trellis {
     tr_params prm;
@@ -398,7 +398,11 @@ int dsdhip_trellis_table_begin(dsdhip_trellis_t *t, int njobs)
  * complete, published in index order).  Every entry must be published, or the launch
  * aborted with dsdhip_trellis_feed_abort, before the stream is joined; a
  * wave that waits longer than limit_s seconds gives up (table_end then
- * reports an error).  Returns 0, 1 when the host-fed form does not apply
+ * reports an error).  The launch waits for the host, so nothing the host
+ * needs before it can publish (input copies it reads, events it waits on)
+ * may be queued after it: streams can share a hardware queue, so queue
+ * such work before this call, on `stream`.  Returns 0, 1 when the host-fed
+ * form does not apply
  * to these parameters (launch the entries with table_launch instead), or
  * -1 on failure.
  */
@@ -431,7 +435,10 @@ int dsdhip_trellis_feed_begin(dsdhip_trellis_t *t, void *stream, double limit_s)
     __atomic_store_n(&t->h_feed->abort, 0, __ATOMIC_RELEASE);
     const uint64_t poll = (uint64_t)khz * 2 / 1000;           /* 2 us */
     const uint64_t limit = (uint64_t)((double)khz * 1000.0 * limit_s);
-    if (tr_launch_feed(&t->prm, hd, t->d_jobs, n, hc, t->d_feed, t->d_ctx, poll ? poll : 1, limit, s) != 0)
+    /* a second bound in wait rounds: a round sleeps about 1000 clocks (at
+     * least 0.3 us), so this is at most a few times limit_s */
+    const uint64_t max_iter = (uint64_t)(limit_s * 3.0e6) + 1;
+    if (tr_launch_feed(&t->prm, hd, t->d_jobs, n, hc, t->d_feed, t->d_ctx, poll ? poll : 1, limit, max_iter, s) != 0)
         return -1;
     t->feed_on = 1;
     return 0;

@@ -622,72 +622,115 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
             wsync();
             s.gtab[gi] = vzero64();
 
-            /* fixed point of early skip + duplicate-path resolution */
-            const u64 LTc = LT & CAND & lowerm;
-            u64 DS = 0, KM = 0;
-            /* ktgt: in the first killer of a registrant, that registrant */
-            int ktgt = 64, R = 64;
-            bool pass = false, kil = false;
-            u64 KILLERS = 0;
-            int it;
-            for (it = 0; it < 64; it++) {
-                /* registrants already removed when this lane is reached:
-                 * those whose first killer comes before it */
-                u64 kb = 0;
-                if (KM) kb = wave_excl_or(ktgt < 64 ? (1ull << ktgt) : 0ull);
-                const int cnt = __popcll(LTc & ~DS & ~kb);
-                pass = cand && cnt < K;
-                const u64 nPASS = ballot(pass);
-                const u64 gp = G & nPASS;
-                R = gp ? ff1(gp) : 64;
-                const bool dup = pass && R != lane;
-                const bool rlt = dup && ((LT >> (R & 63)) & 1);
-                kil = dup && !rlt;
-                const u64 nDS = ballot(rlt);
-                KILLERS = ballot(kil);
-                /* every killer of a registrant is in the registrant's path
-                 * group (R is the group's first passing lane): the group's
-                 * first killer removes it */
-                const u64 gk = G & KILLERS;
-                const u64 nK = ballot(pass && R == lane && gk != 0);
-                const int nkg = (kil && lane == ff1(gk)) ? R : 64;
-                /* the iteration map depends on (DS, KM, ktgt) only.  Removals
-                 * only lower the counts, so every pass set contains the first
-                 * one (computed with no removals): when that first set is all
-                 * of CAND it never changes and this state is the fixed point */
-                const bool same = (it == 0 && nPASS == CAND) ||
-                                  (nDS == DS && nK == KM && ballot(nkg != ktgt) == 0);
-                DS = nDS;
-                KM = nK;
-                ktgt = nkg;
-                if (same) break;
-            }
-            if (it >= 64) slow = true;
-
-            const bool reg = pass && R == lane;
-            const u64 REG = ballot(reg);
-            if (!slow && K == TR_MAXK && nb == 1) {
-                const u64 gk = G & KILLERS;
-                const u64 FIRSTK = ballot(kil && gk && lane == ff1(gk));
-                const u64 INC = REG | (KILLERS & ~FIRSTK);
-                const u64 CL = ballot(reg && mbcnt(INC) >= K);
-                if (CL) {
-                    const int fc = ff1(CL);
-                    if (ballot(pass && (path & 127u) == 0 && lane > fc)) slow = true;
+            /* Closed form where every same-path group holds at most two
+             * candidates (two children share a path only when their
+             * parents' paths differ in the oldest bit alone).  The
+             * survivors are then the first K, in (metric, index) order, of
+             * the candidates whose partner does not precede them in that
+             * order.  The early skip only decides which candidates
+             * register in the hash (the links of the new entries): a
+             * candidate passes when fewer than K earlier candidates still
+             * listed when it is reached precede it; a pair's earlier member
+             * leaves the list at its better partner, a later member whose
+             * partner is better never enters it.  At K = 32 in bank 1 the
+             * write past act[31] can precede a bucket-0 lookup; steps where
+             * that is possible (a passing bucket-0 candidate with more than
+             * K passing ones before it), and groups of three or more, go to
+             * the fixed point below. */
+            bool done = false;
+            if (ballot(cand && __popcll(G) > 2) == 0) {
+                const u64 pm = G & ~(1ull << lane);
+                const int pl = pm ? ff1(pm) : 0;
+                const bool pb = pm && ((LT >> pl) & 1);         /* partner precedes: dropped */
+                const u64 DROP = ballot(pb);
+                const u64 E = ballot(pb && pl < lane);
+                /* this lane takes its earlier, worse partner off the list */
+                const u64 LATE = wave_excl_or((pm && pl < lane && !pb) ? (1ull << pl) : 0ull);
+                const bool ps = cand && __popcll(LT & CAND & lowerm & ~(E | LATE)) < K;
+                const u64 PS = ballot(ps);
+                bool decline = false;
+                if (K == TR_MAXK && nb == 1)
+                    decline = ballot(ps && (path & 127u) == 0 && __popcll(PS & lowerm) > K) != 0;
+                if (!decline) {
+                    const bool rg = ps && !(pm && pl < lane && ((PS >> pl) & 1));
+                    const u64 tw = ballot(rg) & G2;
+                    if (rg) s.link[nb][lane] = (tw & lowerm) ? (int8_t)ff1(tw) : (int8_t)-1;
+                    const u64 F = CAND & ~DROP;
+                    const int rk = __popcll(LT & F);
+                    myrank = (((F >> lane) & 1) && rk < K) ? rk : 255;
+                    const int nf = __popcll(F);
+                    nsurv = nf < K ? nf : K;
+                    done = true;
                 }
             }
-            if (!slow) {
-                /* hash links of the new registrations: the previous
-                 * registrant of the same bucket in index order.  With paths
-                 * below 256 a bucket holds at most two registrants, one for
-                 * each of the paths p and p ^ 128 */
-                const u64 tw = REG & G2;
-                if (reg) s.link[nb][lane] = (tw & lowerm) ? (int8_t)ff1(tw) : (int8_t)-1;
-                const u64 F = CAND & ~DS & ~KM;
-                const int rk = __popcll(LT & F);
-                myrank = (((F >> lane) & 1) && rk < K) ? rk : 255;
-                const int nf = __popcll(F);
-                nsurv = nf < K ? nf : K;
+            if (!done) {
+                /* fixed point of early skip + duplicate-path resolution */
+                const u64 LTc = LT & CAND & lowerm;
+                u64 DS = 0, KM = 0;
+                /* ktgt: in the first killer of a registrant, that registrant */
+                int ktgt = 64, R = 64;
+                bool pass = false, kil = false;
+                u64 KILLERS = 0;
+                int it;
+                for (it = 0; it < 64; it++) {
+                    /* registrants already removed when this lane is reached:
+                     * those whose first killer comes before it */
+                    u64 kb = 0;
+                    if (KM) kb = wave_excl_or(ktgt < 64 ? (1ull << ktgt) : 0ull);
+                    const int cnt = __popcll(LTc & ~DS & ~kb);
+                    pass = cand && cnt < K;
+                    const u64 nPASS = ballot(pass);
+                    const u64 gp = G & nPASS;
+                    R = gp ? ff1(gp) : 64;
+                    const bool dup = pass && R != lane;
+                    const bool rlt = dup && ((LT >> (R & 63)) & 1);
+                    kil = dup && !rlt;
+                    const u64 nDS = ballot(rlt);
+                    KILLERS = ballot(kil);
+                    /* every killer of a registrant is in the registrant's path
+                     * group (R is the group's first passing lane): the group's
+                     * first killer removes it */
+                    const u64 gk = G & KILLERS;
+                    const u64 nK = ballot(pass && R == lane && gk != 0);
+                    const int nkg = (kil && lane == ff1(gk)) ? R : 64;
+                    /* the iteration map depends on (DS, KM, ktgt) only.  Removals
+                     * only lower the counts, so every pass set contains the first
+                     * one (computed with no removals): when that first set is all
+                     * of CAND it never changes and this state is the fixed point */
+                    const bool same = (it == 0 && nPASS == CAND) ||
+                                      (nDS == DS && nK == KM && ballot(nkg != ktgt) == 0);
+                    DS = nDS;
+                    KM = nK;
+                    ktgt = nkg;
+                    if (same) break;
+                }
+                if (it >= 64) slow = true;
+
+                const bool reg = pass && R == lane;
+                const u64 REG = ballot(reg);
+                if (!slow && K == TR_MAXK && nb == 1) {
+                    const u64 gk = G & KILLERS;
+                    const u64 FIRSTK = ballot(kil && gk && lane == ff1(gk));
+                    const u64 INC = REG | (KILLERS & ~FIRSTK);
+                    const u64 CL = ballot(reg && mbcnt(INC) >= K);
+                    if (CL) {
+                        const int fc = ff1(CL);
+                        if (ballot(pass && (path & 127u) == 0 && lane > fc)) slow = true;
+                    }
+                }
+                if (!slow) {
+                    /* hash links of the new registrations: the previous
+                     * registrant of the same bucket in index order.  With paths
+                     * below 256 a bucket holds at most two registrants, one for
+                     * each of the paths p and p ^ 128 */
+                    const u64 tw = REG & G2;
+                    if (reg) s.link[nb][lane] = (tw & lowerm) ? (int8_t)ff1(tw) : (int8_t)-1;
+                    const u64 F = CAND & ~DS & ~KM;
+                    const int rk = __popcll(LT & F);
+                    myrank = (((F >> lane) & 1) && rk < K) ? rk : 255;
+                    const int nf = __popcll(F);
+                    nsurv = nf < K ? nf : K;
+                }
             }
         }
         if (slow) {
@@ -843,20 +886,26 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(TrWave
 
 /* Host-fed queue: the wait of one wave (lane 0) for job j.  Returns 0 once
  * the host has published j, 1 on the host's abort flag or when the wait
- * exceeds `limit` wall-clock ticks (dc->err is then set).  Waves read the
- * device copy of the published count; the host word is read by one wave at
- * a time, at most once per `poll` ticks (a compare-and-swap on the time of
- * the last read), which raises the device copy for every waiting wave. */
-__device__ int feed_wait(const tr_feed_host *hc, tr_feed_dev *dc, int j, uint64_t poll, uint64_t limit)
+ * exceeds `limit` wall-clock ticks or `max_iter` rounds (dc->err is then
+ * set); both bounds are checked on every round.  Waves read the device copy
+ * of the published count; the host word is read by one wave at a time, at
+ * most once per `poll` ticks (a compare-and-swap on the time of the last
+ * read), which raises the device copy for every waiting wave. */
+__device__ int feed_wait(const tr_feed_host *hc, tr_feed_dev *dc, int j, uint64_t poll, uint64_t limit,
+                         uint64_t max_iter)
 {
     const uint64_t t_start = (uint64_t)wall_clock64();
-    for (;;) {
+    for (uint64_t iter = 0;; iter++) {
         const int r = __hip_atomic_load(&dc->ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
         if (r > j &&
             __hip_atomic_load(&hc->ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) > j)
             return 0;
         if (__hip_atomic_load(&dc->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return 1;
         const uint64_t now = (uint64_t)wall_clock64();
+        if ((now > t_start && now - t_start > limit) || iter >= max_iter) {
+            __hip_atomic_fetch_or(&dc->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return 1;
+        }
         uint64_t last = __hip_atomic_load(&dc->last_poll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (now > last && now - last >= poll &&
             __hip_atomic_compare_exchange_strong(&dc->last_poll, &last, now, __ATOMIC_RELAXED,
@@ -869,11 +918,6 @@ __device__ int feed_wait(const tr_feed_host *hc, tr_feed_dev *dc, int j, uint64_
                 return 1;
             }
             if (h > j) return 0;
-            continue;
-        }
-        if (now > t_start && now - t_start > limit) {
-            __hip_atomic_fetch_or(&dc->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return 1;
         }
         __builtin_amdgcn_s_sleep(16);
     }
@@ -895,7 +939,7 @@ static_assert(offsetof(tr_job, slow_steps) == offsetof(tr_job, kept) + 8 &&
 template <int P, int HW>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TrWaves<HW>::value, 8))) void tr_run_feed(
     tr_params prm, const tr_job *hjobs, tr_job *jobs, const tr_feed_host *hc, tr_feed_dev *dc, tr_ctx *ctxs,
-    uint64_t poll, uint64_t limit)
+    uint64_t poll, uint64_t limit, uint64_t max_iter)
 {
     __shared__ TrLds<P, HW> s;
     __shared__ tr_job jd;
@@ -903,7 +947,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TrWaves<HW>:
     const int lane = threadIdx.x;
     const int j = blockIdx.x;
     int st = 0;
-    if (lane == 0) st = feed_wait(hc, dc, j, poll, limit);
+    if (lane == 0) st = feed_wait(hc, dc, j, poll, limit, max_iter);
     if (__builtin_amdgcn_readfirstlane(st)) return;
     /* lane w copies word w of the descriptor into LDS */
     if (lane < NW)
@@ -931,9 +975,9 @@ __global__ void tr_feed_init(tr_feed_dev *dc)
 
 #define TF_INST(P) \
     template __global__ void tr_run_feed<P, 1>(tr_params, const tr_job *, tr_job *, const tr_feed_host *, \
-                                              tr_feed_dev *, tr_ctx *, uint64_t, uint64_t); \
+                                              tr_feed_dev *, tr_ctx *, uint64_t, uint64_t, uint64_t); \
     template __global__ void tr_run_feed<P, 2>(tr_params, const tr_job *, tr_job *, const tr_feed_host *, \
-                                              tr_feed_dev *, tr_ctx *, uint64_t, uint64_t);
+                                              tr_feed_dev *, tr_ctx *, uint64_t, uint64_t, uint64_t);
 TF_INST(4)
 TF_INST(5)
 TF_INST(6)
@@ -1031,10 +1075,13 @@ extern "C" int tr_feed_eligible(const tr_params *prm)
 /* Host-fed launch: resets dc on `stream`, then one one-wave workgroup per
  * entry of h_jobs (coherent pinned host memory, visible to the device at
  * that address) runs it once the host has published it through hc.
- * poll / limit in wall-clock ticks. */
+ * poll / limit in wall-clock ticks, max_iter in wait rounds.  The launch
+ * waits for the host: nothing the host needs to publish may be queued
+ * behind it (streams can share a hardware queue, so queue such work before
+ * the launch, on the same stream). */
 extern "C" int tr_launch_feed(const tr_params *prm, const tr_job *h_jobs, tr_job *d_jobs, int njobs,
                               const tr_feed_host *hc, tr_feed_dev *dc, tr_ctx *d_ctxs, uint64_t poll,
-                              uint64_t limit, hipStream_t stream)
+                              uint64_t limit, uint64_t max_iter, hipStream_t stream)
 {
     if (njobs <= 0) return 0;
     if (!tr_feed_eligible(prm)) return -1;
@@ -1045,7 +1092,7 @@ extern "C" int tr_launch_feed(const tr_params *prm, const tr_job *h_jobs, tr_job
 #define TFL_CASE(PP, HH) \
     if (P == PP && HW == HH) { \
         hipLaunchKernelGGL((tr_run_feed<PP, HH>), grid, block, 0, stream, *prm, h_jobs, d_jobs, hc, dc, d_ctxs, \
-                           poll, limit); \
+                           poll, limit, max_iter); \
         return hipGetLastError() == hipSuccess ? 0 : -1; }
     TFL_CASE(4, 1) TFL_CASE(5, 1) TFL_CASE(6, 1) TFL_CASE(7, 1) TFL_CASE(8, 1)
     TFL_CASE(4, 2) TFL_CASE(5, 2) TFL_CASE(6, 2) TFL_CASE(7, 2) TFL_CASE(8, 2)

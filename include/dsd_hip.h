@@ -338,7 +338,7 @@ int dsdhip_das_estimator_device(dsdhip_das_t *d, int on);
  * the previous one has finished (the form used at K <= 2 and with the FMA
  * filter).  Same bits.  No reference counterpart. */
 int dsdhip_das_batch_launch(dsdhip_das_t *d, int on);
-/* Longest wait, in seconds (default 60), of one wave of the host-fed launch
+/* Longest wait, in seconds (default 20), of one wave of the host-fed launch
  * for its segment; a wave that waits longer leaves and the run returns -1.
  * No reference counterpart. */
 int dsdhip_das_feed_limit(dsdhip_das_t *d, double seconds);

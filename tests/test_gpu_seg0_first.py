@@ -193,3 +193,37 @@ def test_host_fed_wait_limit_ends_the_launch(gpu, oracle):
     d2 = D.Das(fi, B, 16, 32, nch, seg_len=seg, seg0_first=True, reposition=True)
     y, m = d2.run(x)
     assert m == n and np.array_equal(y[:, :n], ref[:, :n])
+
+
+def test_host_fed_launch_with_streams_sharing_queues(gpu, oracle):
+    """Regression: the host-fed launch waits for estimates whose input is
+    copied on the DAS stream; it is queued behind those copies on that
+    stream, so it cannot block them even when streams share hardware queues.
+    With more streams in use than hardware queues, seg0-first runs on the
+    default stream and on a side stream finish bit-exact within a short wait
+    limit."""
+    side = [torch.cuda.Stream() for _ in range(12)]
+    for s in side:
+        with torch.cuda.stream(s):
+            torch.ones(16, device="cuda").sum()
+    torch.cuda.synchronize()
+    nch, F, seg = 2, 150000, 16384
+    x = _pink(nch, F, 95)
+    fi = oracle.ntf_auto(RATE)
+    ref, n, _, _ = ol.das(oracle, fi, B, 16, 32, x, seg_len=seg, nseg=0, seg0_first=1, reposition=1)
+    d_x = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    words = F // 64 + 2
+    for use_side in (False, True):
+        print("case: host-fed launch, device buffers, %s stream, 12 side streams"
+              % ("side" if use_side else "default"), flush=True)
+        d = D.Das(fi, B, 16, 32, nch, seg_len=seg, seg0_first=True, reposition=True)
+        d.feed_limit(5.0)
+        if use_side:
+            d.set_stream(side[5].cuda_stream)
+        d_o = torch.zeros((nch, words), dtype=torch.int64, device="cuda")
+        torch.cuda.synchronize()
+        m = d.run_dev(d_x.data_ptr(), F, F, d_o.data_ptr(), words)
+        torch.cuda.synchronize()
+        w = d_o.cpu().numpy().view(np.uint64)
+        bits = np.unpackbits(w.view(np.uint8), axis=1, bitorder="little")[:, :m]
+        assert m == n and np.array_equal(np.where(bits > 0, 1.0, -1.0).astype(np.float32), ref[:, :n])

@@ -49,3 +49,37 @@ def test_bulk_prune_equals_sequential(model, K, L, B, kind):
     assert steps == n
     assert bad == 0
     assert slow < n // 50          # the literal fallback stays rare
+
+
+@pytest.mark.parametrize("K,L,B,kind", [(32, 64, 8, "pink"), (32, 64, 8, "loud"), (8, 16, 8, "sine"),
+                                       (16, 32, 8, "pink"), (32, 128, 8, "loud"), (4, 8, 8, "sine"),
+                                       (32, 64, 8, "quiet"), (24, 64, 8, "pink")])
+def test_pair_closed_form_equals_sequential(model, K, L, B, kind):
+    """The closed form for groups of at most two same-path candidates (best
+    member of each group, first K in (metric, index) order; registrations
+    from the early skip with pairs leaving the list at their better
+    partner) equals the sequential prune on every step it does not decline,
+    and declines rarely."""
+    rng = np.random.default_rng(7 * K + L)
+    n = 60000
+    t = np.arange(n)
+    if kind == "sine":
+        x = 0.25 * np.sin(2 * np.pi * t * 997 / 11289600 * 16) + 1e-5 * rng.standard_normal(n)
+    elif kind == "loud":
+        x = 0.45 * np.sin(2 * np.pi * t * 997 / 11289600 * 16) + 0.05 * rng.standard_normal(n)
+    elif kind == "quiet":
+        x = 1e-4 * rng.standard_normal(n)
+    else:
+        w = np.fft.irfft(np.fft.rfft(rng.standard_normal(n)) / np.sqrt(np.arange(n // 2 + 1) + 1.0), n)
+        x = np.clip(0.18 * w / np.std(w), -0.5, 0.5)
+    model.lib.pair_model_run.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_size_t,
+                                         C.POINTER(C.c_ulonglong)]
+    c = model.ctx(model.ntf_auto(11289600), B, K, L)
+    st = (C.c_ulonglong * 3)()
+    x = np.ascontiguousarray(x)
+    model.lib.pair_model_run(c.h, x.ctypes.data_as(C.POINTER(C.c_double)), n, st)
+    steps, declined, bad = st[0], st[1], st[2]
+    print("steps %d declined %d mismatches %d" % (steps, declined, bad))
+    assert steps == n
+    assert bad == 0
+    assert declined < n // 5
