@@ -540,9 +540,9 @@ def main():
                               "ns_per_sample_per_chain": round(est_ms * 1e6 / max(1, est_n // NCH), 3),
                               "reference_form_blocks": redo, "batches": nb,
                               "note": "two host threads per channel (index guess; its check against "
-                                      "the reference-order sum, which owns the exact state); the "
-                                      "segments whose estimates have arrived launched as one batch "
-                                      "whenever the previous batch has finished (at least 1/32 of the jobs), the last batch on a second stream as soon as the estimate ends"}
+                                      "the reference-order sum, which owns the exact state); every "
+                                      "segment runs in one host-fed persistent launch whose wave "
+                                      "starts as soon as the estimate for that segment is published"}
         return r, n_out
 
     modes = ["seg0-first", "all-parallel"] if args.seed_mode == "both" else [args.seed_mode]
