@@ -462,6 +462,8 @@ def main():
     ap.add_argument("--seed-mode", choices=["both", "seg0-first", "all-parallel"], default="both",
                     help="headline seeding (seg0-first is the reference's at DSD256); both: "
                          "all-parallel is timed beside it")
+    ap.add_argument("--feed-waves", type=int, default=-1,
+                    help="seg0-first: resident waves per CU of the host-fed launch (-1: library default)")
     ap.add_argument("--lf-steps", type=int, default=5,
                     help="differentiable loop filter fwd+bwd steps (0: skip)")
     ap.add_argument("--lf-check-rows", type=int, default=2)
@@ -502,6 +504,8 @@ def main():
     def run_mode(seg0_first):
         D = das.Das(fi, B, K, L, NCH, seg_len=args.seg_len, seg0_first=seg0_first, reposition=True)
         D.timing(True)
+        if args.feed_waves >= 0:
+            D.feed_waves(args.feed_waves)
 
         def step(timed=False):
             chain.reset()
@@ -536,6 +540,8 @@ def main():
              "trellis_inputs_per_step": k_samples // args.steps}
         if seg0_first:
             est_ms, est_n, redo, nb = D.last_estimate()
+            r["timeline_ms"] = dict(zip(["seg0_done", "estimate_start", "last_published", "estimate_joined",
+                                          "trellis_read", "run_done"], D.last_timeline()))
             r["estimator"] = {"host_chain_ms": round(est_ms, 3), "samples_all_channels": est_n,
                               "ns_per_sample_per_chain": round(est_ms * 1e6 / max(1, est_n // NCH), 3),
                               "reference_form_blocks": redo, "batches": nb,

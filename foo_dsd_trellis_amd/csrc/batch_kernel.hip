@@ -75,9 +75,13 @@ extern "C" __global__ __launch_bounds__(64) void bt_plan(
     }
     int64_t *st = seg_st + (int64_t)t * cfg.smax;
     int64_t *sz = seg_sz + (int64_t)t * cfg.smax;
-    for (int s = lane; s < S; s += 64) {
-        st[s] = (int64_t)s * cfg.seg_len;
-        sz[s] = (s == S - 1) ? F - (int64_t)s * cfg.seg_len : cfg.seg_len;
+    /* sizes by the reference's rule for S segments (src/dsp_plugin.c:1944-1953) */
+    {
+        const int64_t base = S > 0 ? F / S : 0, rem = S > 0 ? F % S : 0;
+        for (int s = lane; s < S; s += 64) {
+            st[s] = (int64_t)s * base + (s < rem ? s : rem);
+            sz[s] = (s == S - 1) ? F - st[s] : base + (s < rem ? 1 : 0);
+        }
     }
     __syncthreads();
     /* quiet-boundary reposition on channel 0 (src/dsp_plugin.c:1963-2034) */

@@ -156,6 +156,7 @@ struct dsdhip_das {
     int est_device;              /* seg0-first estimate on one device lane per channel (timing) */
     int batch_launch;            /* seg0-first: batches of ready segments instead of the host-fed launch */
     double feed_limit_s;         /* host-fed launch: longest wait of one wave for its entry */
+    int feed_waves_cu;           /* host-fed launch: resident waves per CU (0: the kernel's own) */
     /* last run */
     int last_S;
     std::vector<int> last_stitch;
@@ -164,6 +165,11 @@ struct dsdhip_das {
     int64_t last_est_samples;    /* samples estimated (all channels) */
     int64_t last_est_redo;       /* estimator blocks recomputed in the reference form */
     int last_batches;            /* pipelined trellis launches */
+    /* host-side timeline of the last seg0-first run (ms from its start): seg0
+     * done, estimate started, last segment published, estimate threads
+     * joined, trellis results read, run done */
+    std::chrono::steady_clock::time_point t_run0;
+    double last_tl[6];
     DasScratch sc;
 };
 
@@ -242,6 +248,7 @@ static dsdhip_das_t *das_create(const double *a, const double *g, int order, int
     d->est_device = 0;
     d->batch_launch = 0;
     d->feed_limit_s = 20.0;
+    d->feed_waves_cu = 8;
     d->last_est_ms = 0.0;
     d->last_est_samples = 0;
     d->last_est_redo = 0;
@@ -359,7 +366,13 @@ int dsdhip_estimate_boundaries_dev(const double *a, const double *g, int order, 
     return 0;
 }
 
-/* Plan for F samples: returns the segment count; fills start/size if given. */
+/* Plan for F samples: returns the segment count; fills start/size if given.
+ * With a segment length the count is floor(F / seg_len) (at least 1), else
+ * the requested count reduced while segments would be shorter than 4
+ * overlaps (src/dsp_plugin.c:1362-1366); either way the sizes follow the
+ * reference's rule for that count, base + (s < rem) with the last taking
+ * the rest (:1944-1953), so no segment is longer than the others by more
+ * than one sample. */
 int dsdhip_das_plan(const dsdhip_das_t *d, size_t F, size_t *start, size_t *size)
 {
     int S;
@@ -374,19 +387,11 @@ int dsdhip_das_plan(const dsdhip_das_t *d, size_t F, size_t *start, size_t *size
     }
     if (start && size) {
         size_t pos = 0;
-        if (d->seg_len > 0) {
-            for (int s = 0; s < S; s++) {
-                start[s] = pos;
-                size[s] = (s == S - 1) ? F - pos : d->seg_len;
-                pos += size[s];
-            }
-        } else {
-            const size_t base = F / (size_t)S, rem = F % (size_t)S;
-            for (int s = 0; s < S; s++) {
-                start[s] = pos;
-                size[s] = (s == S - 1) ? F - pos : base + ((size_t)s < rem ? 1 : 0);
-                pos += size[s];
-            }
+        const size_t base = F / (size_t)S, rem = F % (size_t)S;
+        for (int s = 0; s < S; s++) {
+            start[s] = pos;
+            size[s] = (s == S - 1) ? F - pos : base + ((size_t)s < rem ? 1 : 0);
+            pos += size[s];
         }
     }
     return S;
@@ -630,6 +635,11 @@ static void est_worker(const dsdhip_das_t *d, std::vector<EstChain *> chains, st
  * on a second stream.
  * Fills kept/slow of every segment >= 1.
  */
+static double ms_since(const dsdhip_das_t *d)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - d->t_run0).count();
+}
+
 static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned long long *segbits)
 {
     const int nch = d->nch, T = d->ntracks;
@@ -669,6 +679,7 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
     d->last_est_ms = 0.0;
     d->last_est_samples = 0;
     d->last_est_redo = 0;
+    d->last_tl[0] = ms_since(d);      /* seg0 done: its best states were read */
     if (total == 0) return 0;
     if (dsdhip_trellis_table_begin(tr, total) != 0) return -1;
     if (staged && (sc.ensure_host(staged) != 0 || DasScratch::grow_events(sc.blk, nblk) != 0)) return -1;
@@ -726,7 +737,7 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
      * never see its entries published) */
     int fed = 1;
     if (rc == 0 && !d->batch_launch) {
-        fed = dsdhip_trellis_feed_begin(tr, d->stream, d->feed_limit_s);
+        fed = dsdhip_trellis_feed_begin(tr, d->stream, d->feed_limit_s, d->feed_waves_cu);
         if (fed < 0) rc = -1;
     }
     const bool feed = fed == 0;
@@ -780,6 +791,7 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
         std::vector<std::thread> th;
         /* a chain's index guess gets a thread of its own where the cores allow */
         const int per = (2 * (unsigned)nw + 1 <= hw) ? 2 : 1;
+        d->last_tl[1] = ms_since(d);
         for (int i = 0; i < nw; i++) th.emplace_back(est_worker, d, part[(size_t)i], &fail, per);
         std::vector<int> taken(nchains, 0);
         for (;;) {
@@ -805,7 +817,9 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
             if (all) break;
             std::this_thread::sleep_for(std::chrono::microseconds(feed ? 50 : 200));
         }
+        d->last_tl[2] = ms_since(d);
         for (auto &x : th) x.join();
+        d->last_tl[3] = ms_since(d);
         if (fail.load()) rc = -1;
         for (size_t i = 0; i < nchains; i++) {
             d->last_est_ms = std::max(d->last_est_ms, chains[i].busy_ms);
@@ -830,6 +844,7 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
     std::vector<int64_t> kp((size_t)total), sl((size_t)total);
     int err = 0;
     if (dsdhip_trellis_table_end(tr, d->stream, kp.data(), sl.data(), &err) != 0) return -1;
+    d->last_tl[4] = ms_since(d);
     if (timing) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, sc.ev_t0, sc.ev_t1) == hipSuccess) dsdhip_trellis_timing_add(tr, ms, nbatch);
@@ -855,6 +870,8 @@ static int run_tracks(dsdhip_das_t *d, const double *const *d_in, const int64_t 
     const size_t O = d->O;
     const int L = d->L;
     const int seg0_first = d->seg0_first;
+    d->t_run0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < 6; i++) d->last_tl[i] = 0.0;
     std::vector<TrackRun> R((size_t)T);
     int Smax = 0;
     for (int t = 0; t < T; t++) {
@@ -1199,6 +1216,7 @@ static int run_tracks(dsdhip_das_t *d, const double *const *d_in, const int64_t 
     }
     d->last_S = R[0].S;
     d->last_stitch = R[0].bp;
+    d->last_tl[5] = ms_since(d);
     return 0;
 }
 
@@ -1291,6 +1309,28 @@ int dsdhip_das_feed_limit(dsdhip_das_t *d, double seconds)
 {
     if (!d || !(seconds >= 0.0) || seconds > 3600.0) return -1;
     d->feed_limit_s = seconds;
+    return 0;
+}
+
+/* Host-side timeline of the last seg0-first run, in ms from the run's
+ * start: seg0 done, estimate threads started, last segment published,
+ * estimate threads joined, trellis results read, run done. */
+int dsdhip_das_last_timeline(const dsdhip_das_t *d, double *ms6)
+{
+    if (!d || !ms6) return -1;
+    for (int i = 0; i < 6; i++) ms6[i] = d->last_tl[i];
+    return 0;
+}
+
+/* Resident waves per CU of the host-fed launch (default 8, two per SIMD;
+ * 0: as many as the kernel's registers allow).  The segments arrive at the
+ * estimator's pace, which a few waves per SIMD keep up with; fewer resident
+ * waves keep the last segments on SIMDs of their own, so the step ends
+ * about one segment's latency after the estimate. */
+int dsdhip_das_feed_waves(dsdhip_das_t *d, int waves_per_cu)
+{
+    if (!d || waves_per_cu < 0 || waves_per_cu > 64) return -1;
+    d->feed_waves_cu = waves_per_cu;
     return 0;
 }
 

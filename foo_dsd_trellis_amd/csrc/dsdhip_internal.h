@@ -30,7 +30,7 @@ int dsdhip_trellis_table_end(dsdhip_trellis_t *t, void *stream, int64_t *kept, i
                              int *err_out);
 /* host-fed launch over the table: one persistent launch whose waves wait for
  * the entries the host publishes (0: running, 1: not applicable, -1 error) */
-int dsdhip_trellis_feed_begin(dsdhip_trellis_t *t, void *stream, double limit_s);
+int dsdhip_trellis_feed_begin(dsdhip_trellis_t *t, void *stream, double limit_s, int waves_per_cu);
 int dsdhip_trellis_feed_publish(dsdhip_trellis_t *t, int n);
 int dsdhip_trellis_feed_abort(dsdhip_trellis_t *t);
 int dsdhip_trellis_timing_add(dsdhip_trellis_t *t, double ms, int64_t launches);

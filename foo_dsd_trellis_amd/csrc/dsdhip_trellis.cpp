@@ -20,7 +20,8 @@ extern "C" int tr_launch_jobs(const tr_params *prm, tr_job *d_jobs, int njobs,
 extern "C" int tr_feed_eligible(const tr_params *prm);
 extern "C" int tr_launch_feed(const tr_params *prm, const tr_job *h_jobs, tr_job *d_jobs, int njobs,
                               const tr_feed_host *hc, tr_feed_dev *dc, tr_ctx *d_ctxs, uint64_t poll,
-                              uint64_t limit, uint64_t max_iter, hipStream_t stream);
+                              uint64_t limit, uint64_t max_iter, size_t lds_pad, hipStream_t stream);
+extern "C" int tr_feed_static_lds(const tr_params *prm);
 
 struct dsdhip_trellis {
     tr_params prm;
@@ -406,7 +407,7 @@ int dsdhip_trellis_table_begin(dsdhip_trellis_t *t, int njobs)
  * to these parameters (launch the entries with table_launch instead), or
  * -1 on failure.
  */
-int dsdhip_trellis_feed_begin(dsdhip_trellis_t *t, void *stream, double limit_s)
+int dsdhip_trellis_feed_begin(dsdhip_trellis_t *t, void *stream, double limit_s, int waves_per_cu)
 {
     hipStream_t s = (hipStream_t)stream;
     if (!t) return -1;
@@ -422,10 +423,21 @@ int dsdhip_trellis_feed_begin(dsdhip_trellis_t *t, void *stream, double limit_s)
         t->d_feed = NULL;
         return -1;
     }
-    int dev = 0, khz = 0;
+    int dev = 0, khz = 0, lds_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0 ||
+        hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess)
         return -1;
+    /* at most waves_per_cu resident workgroups per CU: pad each one's LDS to
+     * just over lds_cu / (waves_per_cu + 1) */
+    size_t pad = 0;
+    if (waves_per_cu > 0) {
+        const int st = tr_feed_static_lds(&t->prm);
+        if (st < 0) return -1;
+        const size_t want = (size_t)lds_cu / (size_t)(waves_per_cu + 1) + 256;
+        if (want > (size_t)st) pad = want - (size_t)st;
+        if ((size_t)st + pad > 65536) pad = 65536 > (size_t)st ? 65536 - (size_t)st : 0;
+    }
     tr_job *hd = NULL;
     tr_feed_host *hc = NULL;
     if (hipHostGetDevicePointer((void **)&hd, t->table, 0) != hipSuccess ||
@@ -438,7 +450,8 @@ int dsdhip_trellis_feed_begin(dsdhip_trellis_t *t, void *stream, double limit_s)
     /* a second bound in wait rounds: a round sleeps about 1000 clocks (at
      * least 0.3 us), so this is at most a few times limit_s */
     const uint64_t max_iter = (uint64_t)(limit_s * 3.0e6) + 1;
-    if (tr_launch_feed(&t->prm, hd, t->d_jobs, n, hc, t->d_feed, t->d_ctx, poll ? poll : 1, limit, max_iter, s) != 0)
+    if (tr_launch_feed(&t->prm, hd, t->d_jobs, n, hc, t->d_feed, t->d_ctx, poll ? poll : 1, limit, max_iter, pad,
+                       s) != 0)
         return -1;
     t->feed_on = 1;
     return 0;

@@ -1075,13 +1075,14 @@ extern "C" int tr_feed_eligible(const tr_params *prm)
 /* Host-fed launch: resets dc on `stream`, then one one-wave workgroup per
  * entry of h_jobs (coherent pinned host memory, visible to the device at
  * that address) runs it once the host has published it through hc.
- * poll / limit in wall-clock ticks, max_iter in wait rounds.  The launch
- * waits for the host: nothing the host needs to publish may be queued
- * behind it (streams can share a hardware queue, so queue such work before
- * the launch, on the same stream). */
+ * poll / limit in wall-clock ticks, max_iter in wait rounds; lds_pad bytes
+ * of dynamic LDS per workgroup (unused) cap the resident workgroups per CU.
+ * The launch waits for the host: nothing the host needs to publish may be
+ * queued behind it (streams can share a hardware queue, so queue such work
+ * before the launch, on the same stream). */
 extern "C" int tr_launch_feed(const tr_params *prm, const tr_job *h_jobs, tr_job *d_jobs, int njobs,
                               const tr_feed_host *hc, tr_feed_dev *dc, tr_ctx *d_ctxs, uint64_t poll,
-                              uint64_t limit, uint64_t max_iter, hipStream_t stream)
+                              uint64_t limit, uint64_t max_iter, size_t lds_pad, hipStream_t stream)
 {
     if (njobs <= 0) return 0;
     if (!tr_feed_eligible(prm)) return -1;
@@ -1091,11 +1092,25 @@ extern "C" int tr_launch_feed(const tr_params *prm, const tr_job *h_jobs, tr_job
     dim3 grid((unsigned)njobs), block(64);
 #define TFL_CASE(PP, HH) \
     if (P == PP && HW == HH) { \
-        hipLaunchKernelGGL((tr_run_feed<PP, HH>), grid, block, 0, stream, *prm, h_jobs, d_jobs, hc, dc, d_ctxs, \
-                           poll, limit, max_iter); \
+        hipLaunchKernelGGL((tr_run_feed<PP, HH>), grid, block, lds_pad, stream, *prm, h_jobs, d_jobs, hc, dc, \
+                           d_ctxs, poll, limit, max_iter); \
         return hipGetLastError() == hipSuccess ? 0 : -1; }
     TFL_CASE(4, 1) TFL_CASE(5, 1) TFL_CASE(6, 1) TFL_CASE(7, 1) TFL_CASE(8, 1)
     TFL_CASE(4, 2) TFL_CASE(5, 2) TFL_CASE(6, 2) TFL_CASE(7, 2) TFL_CASE(8, 2)
+    return -1;
+}
+
+/* Static LDS bytes of the host-fed kernel for prm (-1 if not covered). */
+extern "C" int tr_feed_static_lds(const tr_params *prm)
+{
+    const int P = prm->P, HW = prm->L > 64 ? 2 : 1;
+    hipFuncAttributes fa;
+#define TFS_CASE(PP, HH) \
+    if (P == PP && HW == HH) { \
+        if (hipFuncGetAttributes(&fa, (const void *)tr_run_feed<PP, HH>) != hipSuccess) return -1; \
+        return (int)fa.sharedSizeBytes; }
+    TFS_CASE(4, 1) TFS_CASE(5, 1) TFS_CASE(6, 1) TFS_CASE(7, 1) TFS_CASE(8, 1)
+    TFS_CASE(4, 2) TFS_CASE(5, 2) TFS_CASE(6, 2) TFS_CASE(7, 2) TFS_CASE(8, 2)
     return -1;
 }
 

@@ -144,6 +144,19 @@ class Das:
         a wave that waits longer leaves and the run fails."""
         check(lib().dsdhip_das_feed_limit(self.h, float(seconds)), "feed_limit")
 
+    def feed_waves(self, waves_per_cu):
+        """Resident waves per CU of the host-fed launch (0: the kernel's own
+        limit)."""
+        check(lib().dsdhip_das_feed_waves(self.h, int(waves_per_cu)), "feed_waves")
+
+    def last_timeline(self):
+        """Host-side timeline of the last seg0-first run (ms from its start):
+        seg0 done, estimate started, last segment published, estimate joined,
+        trellis results read, run done."""
+        t = (C.c_double * 6)()
+        check(lib().dsdhip_das_last_timeline(self.h, t), "last_timeline")
+        return [round(v, 3) for v in t]
+
     def last_estimate(self):
         """(longest estimator chain ms, samples estimated, redo blocks, batches)
         of the last seg0-first run."""

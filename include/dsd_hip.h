@@ -342,6 +342,15 @@ int dsdhip_das_batch_launch(dsdhip_das_t *d, int on);
  * for its segment; a wave that waits longer leaves and the run returns -1.
  * No reference counterpart. */
 int dsdhip_das_feed_limit(dsdhip_das_t *d, double seconds);
+/* Resident waves per CU of the host-fed launch (default 8, i.e. two per
+ * SIMD; 0: as many as the kernel's registers allow).  No reference
+ * counterpart. */
+int dsdhip_das_feed_waves(dsdhip_das_t *d, int waves_per_cu);
+/* Host-side timeline of the last seg0-first run, ms from the run's start:
+ * seg0 done, estimate threads started, last segment published, estimate
+ * threads joined, trellis results read, run done.  No reference
+ * counterpart. */
+int dsdhip_das_last_timeline(const dsdhip_das_t *d, double *ms6);
 /* Estimator figures of the last seg0-first run: longest chain's busy time,
  * samples estimated over all channels, blocks the fast form handed back to
  * the reference form, pipelined trellis launches. */

@@ -44,20 +44,14 @@ int or_das_plan_init(or_das_plan_t *p, size_t F, size_t seg_len, int nseg,
     p->reposition = reposition;
     p->nom_start = (size_t *)calloc((size_t)S, sizeof(size_t));
     p->nom_size = (size_t *)calloc((size_t)S, sizeof(size_t));
+    /* sizes by the reference's rule for S segments (src/dsp_plugin.c:1944-
+     * 1953); a segment length only sets the count */
     size_t pos = 0;
-    if (seg_len > 0) {
-        for (int s = 0; s < S; s++) {
-            p->nom_start[s] = pos;
-            p->nom_size[s] = (s == S - 1) ? F - pos : seg_len;
-            pos += p->nom_size[s];
-        }
-    } else {
-        size_t base = F / (size_t)S, rem = F % (size_t)S;
-        for (int s = 0; s < S; s++) {
-            p->nom_start[s] = pos;
-            p->nom_size[s] = (s == S - 1) ? F - pos : base + ((size_t)s < rem ? 1 : 0);
-            pos += p->nom_size[s];
-        }
+    size_t base = F / (size_t)S, rem = F % (size_t)S;
+    for (int s = 0; s < S; s++) {
+        p->nom_start[s] = pos;
+        p->nom_size[s] = (s == S - 1) ? F - pos : base + ((size_t)s < rem ? 1 : 0);
+        pos += p->nom_size[s];
     }
     return S;
 }
