@@ -22,6 +22,9 @@ def _sigs(L):
     L.dsdhip_conv_partitions.argtypes = [C.c_void_p]
     L.dsdhip_conv_process_dev.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_size_t, C.c_void_p,
                                           C.c_int64]
+    L.dsdhip_conv_process_fifo_dev.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_size_t, C.c_void_p,
+                                               C.c_int64]
+    L.dsdhip_conv_fifo_state.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L._conv_sigs = True
 
 
@@ -49,8 +52,21 @@ class Upols:
         _lib.check(self.lib.dsdhip_conv_set_stream(self.h, stream), "dsdhip_conv_set_stream")
 
     def process_dev(self, d_in, in_stride, n, d_out, out_stride):
+        """n a multiple of P."""
         _lib.check(self.lib.dsdhip_conv_process_dev(self.h, d_in, in_stride, n, d_out, out_stride),
                    "dsdhip_conv_process_dev")
+
+    def process_fifo_dev(self, d_in, in_stride, n, d_out, out_stride):
+        """Any n, with the reference's FIFO (conv_process_direct,
+        src/convolution.c:893-958)."""
+        _lib.check(self.lib.dsdhip_conv_process_fifo_dev(self.h, d_in, in_stride, n, d_out, out_stride),
+                   "dsdhip_conv_process_fifo_dev")
+
+    def fifo_state(self):
+        """(pending input samples, produced samples waiting) per channel."""
+        a, b = C.c_int64(0), C.c_int64(0)
+        _lib.check(self.lib.dsdhip_conv_fifo_state(self.h, C.byref(a), C.byref(b)), "dsdhip_conv_fifo_state")
+        return a.value, b.value
 
     def __del__(self):
         try:

@@ -47,11 +47,18 @@ extern "C" __global__ void cv_tail(const double *__restrict__ in, int64_t in_str
 // of the chunk.  hist: [c][np-1][K], X: [c][nb][K], H: [c][np][K], Y:
 // [c][nb][K] (float2 = complex), K = P+1 bins.  One thread per (bin, tile of
 // CV_TB blocks, channel); partitions in passes of CV_TB with a window of
-// 2*CV_TB-1 spectra in registers.  np and nb are multiples of CV_TB.
+// 2*CV_TB-1 spectra in registers.  np is a multiple of CV_TB; nb is any
+// count (a partial last tile reads zeros past the chunk and stores only the
+// chunk's blocks).
 __device__ __forceinline__ float2 cv_S(const float2 *__restrict__ hist, const float2 *__restrict__ X,
                                        int idx, int nh, int K, int k)
 {
     return idx < nh ? hist[(int64_t)idx * K + k] : X[(int64_t)(idx - nh) * K + k];
+}
+__device__ __forceinline__ float2 cv_Sb(const float2 *__restrict__ hist, const float2 *__restrict__ X,
+                                        int idx, int nh, int nall, int K, int k)
+{
+    return idx < nall ? cv_S(hist, X, idx, nh, K, k) : make_float2(0.0f, 0.0f);
 }
 
 extern "C" __global__ __launch_bounds__(256) void cv_mac(const float2 *__restrict__ hist,
@@ -75,7 +82,7 @@ extern "C" __global__ __launch_bounds__(256) void cv_mac(const float2 *__restric
         float2 w[2 * CV_TB - 1];
         const int base = nh + b0 - p0 - (CV_TB - 1);
 #pragma unroll
-        for (int u = 0; u < 2 * CV_TB - 1; u++) w[u] = cv_S(hc, xc, base + u, nh, K, k);
+        for (int u = 0; u < 2 * CV_TB - 1; u++) w[u] = cv_Sb(hc, xc, base + u, nh, nh + nb, K, k);
 #pragma unroll
         for (int q = 0; q < CV_TB; q++) {
             const float2 h = Hc[(int64_t)(p0 + q) * K + k];
@@ -92,7 +99,8 @@ extern "C" __global__ __launch_bounds__(256) void cv_mac(const float2 *__restric
     }
     float2 *yc = Y + (int64_t)c * nb * K;
 #pragma unroll
-    for (int t = 0; t < CV_TB; t++) yc[(int64_t)(b0 + t) * K + k] = make_float2(ar[t], ai[t]);
+    for (int t = 0; t < CV_TB; t++)
+        if (b0 + t < nb) yc[(int64_t)(b0 + t) * K + k] = make_float2(ar[t], ai[t]);
 }
 
 // FDL history for the next chunk: the last np-1 spectra of [hist | X].
@@ -120,4 +128,17 @@ extern "C" __global__ void cv_unpack(const float *__restrict__ frames, int P, in
     const int b = (int)(i / P), j = (int)(i - (int64_t)b * P);
     const float v = frames[((int64_t)c * nb + b) * 2 * P + P + j];
     out[(int64_t)c * out_stride + i] = (double)v * (1.0 / (2.0 * (double)P));
+}
+
+// ---- FIFO form for any chunk length (src/convolution.c:893-958) ----------
+// Gather of two sources by position: out[c][i] = i < na ? a[c][i] : b[c][i - na]
+// for i < n (rows of the given strides).
+extern "C" __global__ void cv_cat2(const double *__restrict__ a, int64_t a_stride, int64_t na,
+                                   const double *__restrict__ b, int64_t b_stride, int64_t n, int nch,
+                                   double *__restrict__ out, int64_t out_stride)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= n || c >= nch) return;
+    out[(int64_t)c * out_stride + i] = i < na ? a[(int64_t)c * a_stride + i] : b[(int64_t)c * b_stride + (i - na)];
 }

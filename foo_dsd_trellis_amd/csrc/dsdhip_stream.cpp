@@ -25,7 +25,7 @@ struct dsdhip_stream {
     dsdhip_stream_cfg_t cfg;
     int R;                 /* upsampling ratio 2^stages (1 same-rate) */
     size_t cap;            /* samples per channel per chunk at the output rate */
-    int conv_block;        /* 8*P when convolving, else 1 */
+    int conv_block;        /* chunk granularity the chain needs (1: any length) */
     dsdhip_fir_t *fir;     /* NULL same-rate */
     dsdhip_cond_t *cond;   /* boxcar / lowpass state, NULL when unused */
     dsdhip_conv_t *conv;
@@ -97,8 +97,8 @@ dsdhip_stream_t *dsdhip_stream_create_cfg(const dsdhip_stream_cfg_t *cfg)
     }
     if (ok && cfg->ir) {
         ok = (s->conv = dsdhip_conv_create(nch, cfg->P, cfg->ir, cfg->ir_len, cfg->ir_stride)) != nullptr;
-        s->conv_block = 8 * cfg->P;
-        ok = ok && s->cap % (size_t)s->conv_block == 0;
+        /* any chunk length: the convolution's FIFO form */
+        s->conv_block = 1;
     }
     /* seeding: the reference's rule by output rate (src/dsp_plugin.c:2185) */
     const int seg0_first = cfg->seeding == 1 || (cfg->seeding == 0 && cfg->fs_out < DSDHIP_DSD512_RATE);
@@ -198,13 +198,14 @@ static int finish(dsdhip_stream_t *s, size_t n, unsigned long long *d_bits, int6
     if (s->cfg.soft_clip && s->R > 1 &&
         dsdhip_soft_clip_dev(s->d_fir, st, s->cfg.nch, n, s->stream) != 0)
         return -1;
-    if (s->conv && dsdhip_conv_process_dev(s->conv, s->d_fir, st, n, s->d_sig, st) != 0) return -1;
+    if (s->conv && dsdhip_conv_process_fifo_dev(s->conv, s->d_fir, st, n, s->d_sig, st) != 0) return -1;
     s->last_n = n;
     return dsdhip_das_run_dev(s->das, s->d_sig, st, n, d_bits, out_words, out_count);
 }
 
-/* frames PCM frames per channel (<= chunk_frames; frames * ratio a multiple
- * of 8*P when convolving).  d_bits: nch rows of out_words packed words. */
+/* frames PCM frames per channel (<= chunk_frames; with a room IR any length,
+ * through the convolution's FIFO form).  d_bits: nch rows of out_words
+ * packed words. */
 int dsdhip_stream_process_dev(dsdhip_stream_t *s, const float *d_pcm, int64_t pcm_stride, size_t frames,
                               unsigned long long *d_bits, int64_t out_words, size_t *out_count)
 {

@@ -414,7 +414,7 @@ void gpu_hip_plan_destroy(gpu_hip_plan_t *p);
  *    conv_init / conv_load_ir / conv_process (src/convolution.c:530-1041)
  *    for a given IR per channel at the signal rate: partitions of P
  *    samples, FFT size 2P, complex FP32 spectra.  Input and output are
- *    device f64 [nch][stride]; chunks are multiples of 8*P samples.
+ *    device f64 [nch][stride].
  * ====================================================================== */
 typedef struct dsdhip_conv dsdhip_conv_t;
 /* ir: host f64 [nch][ir_stride], ir_len taps per channel. */
@@ -424,9 +424,21 @@ void dsdhip_conv_destroy(dsdhip_conv_t *c);
 int dsdhip_conv_reset(dsdhip_conv_t *c);
 int dsdhip_conv_set_stream(dsdhip_conv_t *c, void *hip_stream);
 int dsdhip_conv_partitions(const dsdhip_conv_t *c);
-/* y = x * h for the next n samples of every channel (stream-ordered). */
+/* y = x * h for the next n samples of every channel, n a multiple of P
+ * (stream-ordered); -1 while the FIFO form below holds samples. */
 int dsdhip_conv_process_dev(dsdhip_conv_t *c, const double *d_in, int64_t in_stride, size_t n,
                             double *d_out, int64_t out_stride);
+/* Any n (replaces conv_process_direct, src/convolution.c:893-958): the
+ * chunk joins the pending input, each whole block of P samples is
+ * convolved and its outputs join those waiting; exactly n waiting samples
+ * are written to d_out, or, when fewer are waiting (no latency is
+ * inserted), all of them followed by the chunk's own input at the rest of
+ * the positions, with nothing carried over.  d_out may equal d_in.  Chunks
+ * that are multiples of P give dsdhip_conv_process_dev's output. */
+int dsdhip_conv_process_fifo_dev(dsdhip_conv_t *c, const double *d_in, int64_t in_stride, size_t n,
+                                 double *d_out, int64_t out_stride);
+/* Samples the FIFO form holds: pending input, produced output waiting. */
+int dsdhip_conv_fifo_state(const dsdhip_conv_t *c, int64_t *pending_in, int64_t *waiting_out);
 
 /* ======================================================================
  * 7. Differentiable loop filter and pre-emphasis (training).
