@@ -77,6 +77,7 @@ _SIGS = {
     "dsdhip_fir_set_stream": ([_vp, _vp], C.c_int),
     "dsdhip_fir_num_stages": ([_vp], C.c_int),
     "dsdhip_fir_reset": ([_vp], C.c_int),
+    "dsdhip_fir_set_fused": ([_vp, C.c_int], C.c_int),
     "dsdhip_fir_process_dev": ([_vp, _vp, C.c_int64, C.c_size_t, _vp, C.c_int64], C.c_int),
     "dsdhip_fir_process": ([_vp, _fp, C.c_size_t, _dp], C.c_int),
     "dsdhip_das_create": ([C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_size_t,

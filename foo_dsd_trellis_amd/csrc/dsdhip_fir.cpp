@@ -20,6 +20,10 @@ extern "C" int fir_launch_stage(const void *x, int in_f32, int64_t x_stride, con
                                 int64_t y_stride, double gain, int apply_gain, hipStream_t stream);
 extern "C" int fir_launch_hist_update(const void *x, int in_f32, int64_t x_stride, int64_t n,
                                       int nch, double *hist, int H, hipStream_t stream);
+extern "C" int fir_launch_pair(const void *x, int in_f32, int64_t x_stride, const double *histA, int64_t n1,
+                               int nch, const double *d_h2A, const double *d_h2B, double *histB,
+                               double *hist_tmp, double *y, int64_t y_stride, double gain, int apply_gain,
+                               hipStream_t stream);
 
 #define FIR_MAX_STAGES 9
 
@@ -36,6 +40,8 @@ struct dsdhip_fir {
     float *d_hin;     /* host-buffer entry: staged input / output, grow-only */
     double *d_hout;
     size_t hin_cap, hout_cap;
+    int fused;        /* adjacent 63-tap stages in one pass (default) */
+    double *d_htmp;   /* nch x 31: the intermediate history of a fused pair */
 };
 
 /* Modified Bessel I0 by its power series, 25 terms with early exit
@@ -99,6 +105,11 @@ static dsdhip_fir_t *fir_alloc(int nst, const int *ntaps, const double *const *h
     f->nst = nst;
     f->nch = nch;
     f->gain = gain;
+    f->fused = 1;
+    if (hipMalloc(&f->d_htmp, sizeof(double) * 31 * (size_t)nch) != hipSuccess) {
+        free(f);
+        return NULL;
+    }
     for (int s = 0; s < nst; s++) {
         const int T = ntaps[s];
         if (T != 63 && T != 127) { dsdhip_fir_destroy(f); return NULL; }
@@ -146,6 +157,7 @@ void dsdhip_fir_destroy(dsdhip_fir_t *f)
         if (f->d_h2[s]) hipFree(f->d_h2[s]);
         if (f->d_hist[s]) hipFree(f->d_hist[s]);
     }
+    if (f->d_htmp) hipFree(f->d_htmp);
     for (int b = 0; b < 2; b++) if (f->d_buf[b]) hipFree(f->d_buf[b]);
     if (f->d_hin) hipFree(f->d_hin);
     if (f->d_hout) hipFree(f->d_hout);
@@ -193,6 +205,9 @@ int dsdhip_fir_reserve(dsdhip_fir_t *f, size_t n)
     return 0;
 }
 
+/* Stages in order; two adjacent 63-tap stages run as one pass (the
+ * intermediate stream stays in LDS) unless fusion is off.  Every output is
+ * the same sum in the same order either way. */
 static int fir_run(dsdhip_fir_t *f, const void *d_in, int in_f32, int64_t in_stride, size_t n, double *d_out,
                    int64_t out_stride)
 {
@@ -203,23 +218,40 @@ static int fir_run(dsdhip_fir_t *f, const void *d_in, int in_f32, int64_t in_str
     int src_f32 = in_f32;
     int64_t src_stride = in_stride;
     size_t cnt = n;
-    for (int s = 0; s < f->nst; s++) {
-        const bool last = s == f->nst - 1;
-        double *dst = last ? d_out : f->d_buf[s & 1];
-        const int64_t dst_stride = last ? out_stride : (int64_t)(2 * cnt);
+    int nb = 0;   /* next ping-pong buffer */
+    for (int s = 0; s < f->nst;) {
+        const bool pair = f->fused && s + 1 < f->nst && f->ntaps[s] == 63 && f->ntaps[s + 1] == 63;
+        const int k = pair ? 2 : 1;
+        const bool last = s + k == f->nst;
+        const int64_t nout = (int64_t)cnt << k;
+        double *dst = last ? d_out : f->d_buf[nb];
+        const int64_t dst_stride = last ? out_stride : nout;
         const int H = (f->ntaps[s] + 1) / 2 - 1;
-        if (fir_launch_stage(src, src_f32, src_stride, f->d_hist[s], (int64_t)cnt, f->nch,
-                             f->ntaps[s], f->d_h2[s], dst, dst_stride, f->gain,
-                             (last && f->gain != 1.0) ? 1 : 0, f->stream) != 0)
+        const int rc = pair ? fir_launch_pair(src, src_f32, src_stride, f->d_hist[s], (int64_t)cnt, f->nch, f->d_h2[s],
+                                              f->d_h2[s + 1], f->d_hist[s + 1], f->d_htmp, dst, dst_stride, f->gain,
+                                              (last && f->gain != 1.0) ? 1 : 0, f->stream)
+                            : fir_launch_stage(src, src_f32, src_stride, f->d_hist[s], (int64_t)cnt, f->nch,
+                                               f->ntaps[s], f->d_h2[s], dst, dst_stride, f->gain,
+                                               (last && f->gain != 1.0) ? 1 : 0, f->stream);
+        if (rc != 0) return -1;
+        if (fir_launch_hist_update(src, src_f32, src_stride, (int64_t)cnt, f->nch, f->d_hist[s], H, f->stream) != 0)
             return -1;
-        if (fir_launch_hist_update(src, src_f32, src_stride, (int64_t)cnt, f->nch, f->d_hist[s], H,
-                                   f->stream) != 0)
-            return -1;
+        if (!last) nb ^= 1;
         src = dst;
         src_f32 = 0;
         src_stride = dst_stride;
-        cnt *= 2;
+        cnt = (size_t)nout;
+        s += k;
     }
+    return 0;
+}
+
+/* Fused passes of adjacent 63-tap stages on (default) or off (one pass per
+ * stage); the outputs are identical. */
+int dsdhip_fir_set_fused(dsdhip_fir_t *f, int on)
+{
+    if (!f) return -1;
+    f->fused = on ? 1 : 0;
     return 0;
 }
 

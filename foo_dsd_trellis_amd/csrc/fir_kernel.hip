@@ -119,7 +119,173 @@ __global__ void fir_hist_update(const TIn *__restrict__ x, int64_t x_stride, int
     for (int i = threadIdx.x; i < H; i += blockDim.x) hc[i] = tmp[i];
 }
 
+/*
+ * Two consecutive 63-tap 2x stages in one pass: a block takes FU_T inputs
+ * of the first stage (plus the 47 before them), computes the first stage's
+ * 2*FU_T outputs and the 31 before them (the second stage's history) into
+ * LDS, and writes only the second stage's 4*FU_T outputs.  Each output is
+ * the same ascending-q FMA sum as fir_up2_kernel, so the result is
+ * identical to running the two stages one after the other.  Block 0 takes
+ * the 31 intermediate samples before the chunk from histB (the second
+ * stage's delay line); the last block stores the chunk's last 31
+ * intermediate samples into histB_next (copied into histB afterwards, so no
+ * block reads a line another block writes).
+ */
+constexpr int FU_T = 512;           /* first-stage inputs per block */
+constexpr int FU_H = 31;            /* history of a 63-tap stage */
+constexpr int FU_XH = FU_H + 16;    /* inputs before the tile: 16 for the halo pairs, 31 for their taps */
+
+template <typename TIn>
+__global__ __launch_bounds__(FIR_THREADS) void fir_up4_kernel(
+    const TIn *__restrict__ x, int64_t x_stride, const double *__restrict__ histA, int64_t n1,
+    const double *__restrict__ h2A, const double *__restrict__ h2B, const double *__restrict__ histB,
+    double *__restrict__ histB_next, double *__restrict__ y, int64_t y_stride, double gain, int apply_gain)
+{
+    constexpr int Q0 = 32, Q1 = 31;
+    constexpr int NI = FU_T + 16;          /* first-stage positions computed (halo pairs first) */
+    __shared__ double xin[FU_T + FU_XH];   /* positions m0 - 47 .. m0 + FU_T - 1 */
+    __shared__ double inter[2 * NI];       /* intermediate positions 2 m0 - 32 .. 2 m0 + 2 FU_T - 1 */
+    __shared__ double2 ost[2 * FU_T];
+
+    const int ch = blockIdx.y;
+    const TIn *xc = x + (int64_t)ch * x_stride;
+    const double *hA = histA ? histA + (int64_t)ch * FU_H : nullptr;
+    const double *hB = histB ? histB + (int64_t)ch * FU_H : nullptr;
+    double *yc = y + (int64_t)ch * y_stride;
+    const int64_t m0 = (int64_t)blockIdx.x * FU_T;
+    const int64_t n2 = 2 * n1;
+
+    for (int i = threadIdx.x; i < FU_T + FU_XH; i += FIR_THREADS) {
+        const int64_t g = m0 - FU_XH + i;
+        double v = 0.0;
+        if (g >= 0) {
+            if (g < n1) v = (double)xc[g];
+        } else if (g >= -FU_H && hA) {
+            v = hA[FU_H + g];
+        }
+        xin[i] = v;
+    }
+    if (m0 == 0) {
+        /* intermediate positions -32 .. -1: the second stage's delay line */
+        for (int k = threadIdx.x; k < 32; k += FIR_THREADS) inter[k] = (k >= 1 && hB) ? hB[k - 1] : 0.0;
+    }
+    __syncthreads();
+
+    /* first stage, two positions per thread and round: position
+     * j = m0 - 16 + li, taps over xin[li + 31 - q] */
+    for (int pr = threadIdx.x; pr < NI / 2; pr += FIR_THREADS) {
+        const int li0 = 2 * pr;
+        if (m0 == 0 && li0 < 16) continue;     /* halo of block 0 comes from histB */
+        double acc0[2], acc1[2], w[2];
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            acc0[r] = 0.0;
+            acc1[r] = 0.0;
+            w[r] = xin[li0 + r + FU_H];
+        }
+#pragma unroll
+        for (int q = 0; q < Q0; q++) {
+            const double he = h2A[2 * q];
+            const double ho = (q < Q1) ? h2A[2 * q + 1] : 0.0;
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                acc0[r] = __builtin_fma(he, w[r], acc0[r]);
+                if (q < Q1) acc1[r] = __builtin_fma(ho, w[r], acc1[r]);
+            }
+            if (q + 1 < Q0) {
+                w[1] = w[0];
+                w[0] = xin[li0 + FU_H - (q + 1)];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            inter[2 * (li0 + r)] = acc0[r];
+            inter[2 * (li0 + r) + 1] = acc1[r];
+        }
+    }
+    __syncthreads();
+
+    /* the chunk's last 31 intermediate samples for the next call */
+    if (histB_next && blockIdx.x == gridDim.x - 1 && threadIdx.x < FU_H) {
+        const int64_t pos = n2 - FU_H + threadIdx.x;
+        histB_next[(int64_t)ch * FU_H + threadIdx.x] = inter[pos - (2 * m0 - 32)];
+    }
+
+    /* second stage: thread t takes intermediate positions 2 m0 + 4t + r */
+    const int base = FIR_R * threadIdx.x + 32;
+    double acc0[FIR_R], acc1[FIR_R], w[FIR_R];
+#pragma unroll
+    for (int r = 0; r < FIR_R; r++) {
+        acc0[r] = 0.0;
+        acc1[r] = 0.0;
+        w[r] = inter[base + r];
+    }
+#pragma unroll
+    for (int q = 0; q < Q0; q++) {
+        const double he = h2B[2 * q];
+        const double ho = (q < Q1) ? h2B[2 * q + 1] : 0.0;
+#pragma unroll
+        for (int r = 0; r < FIR_R; r++) {
+            acc0[r] = __builtin_fma(he, w[r], acc0[r]);
+            if (q < Q1) acc1[r] = __builtin_fma(ho, w[r], acc1[r]);
+        }
+        if (q + 1 < Q0) {
+#pragma unroll
+            for (int r = FIR_R - 1; r > 0; r--) w[r] = w[r - 1];
+            w[0] = inter[base - (q + 1)];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < FIR_R; r++) {
+        double e = acc0[r], o = acc1[r];
+        if (apply_gain) {
+            e = e * gain;
+            o = o * gain;
+        }
+        double2 v;
+        v.x = e;
+        v.y = o;
+        ost[FIR_R * threadIdx.x + r] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < FIR_R; r++) {
+        const int i = r * FIR_THREADS + threadIdx.x;
+        const int64_t p = 2 * m0 + i;
+        if (p < n2) *reinterpret_cast<double2 *>(yc + 2 * p) = ost[i];
+    }
+}
+
+__global__ void fir_copy(const double *__restrict__ src, double *__restrict__ dst, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 } // namespace
+
+/* Two 63-tap stages in one launch over nch channels: n1 first-stage inputs
+ * -> 4 n1 outputs.  histA / histB: the two stages' delay lines (nch x 31);
+ * histB is refreshed from the chunk's intermediate samples through
+ * hist_tmp (nch x 31 scratch). */
+extern "C" int fir_launch_pair(const void *x, int in_f32, int64_t x_stride, const double *histA, int64_t n1,
+                               int nch, const double *d_h2A, const double *d_h2B, double *histB,
+                               double *hist_tmp, double *y, int64_t y_stride, double gain, int apply_gain,
+                               hipStream_t stream)
+{
+    if (n1 <= 0 || nch <= 0) return 0;
+    dim3 grid((unsigned)((n1 + FU_T - 1) / FU_T), (unsigned)nch), block(FIR_THREADS);
+    if (in_f32)
+        hipLaunchKernelGGL((fir_up4_kernel<float>), grid, block, 0, stream, (const float *)x, x_stride, histA, n1,
+                           d_h2A, d_h2B, histB, hist_tmp, y, y_stride, gain, apply_gain);
+    else
+        hipLaunchKernelGGL((fir_up4_kernel<double>), grid, block, 0, stream, (const double *)x, x_stride, histA,
+                           n1, d_h2A, d_h2B, histB, hist_tmp, y, y_stride, gain, apply_gain);
+    if (hipGetLastError() != hipSuccess) return -1;
+    const int nh = nch * FU_H;
+    hipLaunchKernelGGL(fir_copy, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, stream, hist_tmp, histB, nh);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 
 /* Launch one stage over nch channels.  in_f32 selects a float input (first
  * stage).  hist: nch * (Q0-1) doubles or NULL (zero history). */

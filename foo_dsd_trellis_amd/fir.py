@@ -58,6 +58,11 @@ class FirChain:
     def set_stream(self, s):
         check(lib().dsdhip_fir_set_stream(self.h, s), "fir set_stream")
 
+    def set_fused(self, on=True):
+        """Adjacent 63-tap stages in one pass (default) or one pass per stage;
+        identical outputs."""
+        check(lib().dsdhip_fir_set_fused(self.h, 1 if on else 0), "fir set_fused")
+
     def reset(self):
         check(lib().dsdhip_fir_reset(self.h), "fir reset")
 

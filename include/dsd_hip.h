@@ -267,6 +267,10 @@ int dsdhip_fir_set_stream(dsdhip_fir_t *f, void *hip_stream);
 int dsdhip_fir_num_stages(const dsdhip_fir_t *f);
 /* replaces fir_chain_reset (include/fir.h:76): zero delay lines */
 int dsdhip_fir_reset(dsdhip_fir_t *f);
+/* Adjacent 63-tap stages in one pass with the intermediate stream kept in
+ * LDS (on = 1, the default) or one pass per stage (on = 0); identical
+ * outputs.  No reference counterpart. */
+int dsdhip_fir_set_fused(dsdhip_fir_t *f, int on);
 /* replaces fir_chain_process_d (include/fir.h:70-72) for nch channels;
  * delay lines persist between calls.  Device buffers, stream-ordered. */
 int dsdhip_fir_process_dev(dsdhip_fir_t *f, const float *d_in, int64_t in_stride, size_t n,

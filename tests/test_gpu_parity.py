@@ -270,3 +270,33 @@ def test_das_chunks_of_changing_size(gpu, oracle):
         want = ref.chunk(x, seg, 0, 1)
         assert m == want.shape[1], k
         assert np.array_equal(y[:, :m], want), k
+
+
+@pytest.mark.parametrize("fs_in,fs_out", [(44100, 11289600), (48000, 24576000), (44100, 2822400)])
+def test_fir_fused_pairs_equal_stage_by_stage(gpu, oracle, fs_in, fs_out):
+    """Adjacent 63-tap stages fused into one pass give exactly the
+    stage-by-stage outputs and the oracle's, over chunks that cross tile
+    edges, shorter than the fused pass's halo (16 inputs), and carry the
+    delay lines of every stage (the intermediate one included) across
+    chunks."""
+    from foo_dsd_trellis_amd import fir, signals
+    sizes = [1000, 7, 513, 2048, 15, 1, 4096, 333]
+    n = sum(sizes)
+    x = signals.pink_multitone(n, fs=fs_in, nch=2, seed=21)
+    R = fs_out // fs_in
+    outs = []
+    for fused in (True, False):
+        ch = fir.FirChain(fs_in, fs_out, fir.PCM_FIR_GAIN, 2)
+        ch.set_fused(fused)
+        parts = []
+        c0 = 0
+        for m in sizes:
+            print("case: %d -> %d fused=%s chunk %d" % (fs_in, fs_out, fused, m), flush=True)
+            parts.append(ch.process(np.ascontiguousarray(x[:, c0:c0 + m])))
+            c0 += m
+        outs.append(np.concatenate(parts, axis=1))
+    assert outs[0].shape == (2, n * R)
+    assert np.array_equal(outs[0], outs[1])
+    for c in range(2):
+        ref = ol.fir_chain(oracle, fs_in, fs_out, x[c], fir.PCM_FIR_GAIN)
+        assert np.array_equal(outs[0][c], ref), c
