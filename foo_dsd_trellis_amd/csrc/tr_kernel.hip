@@ -449,11 +449,16 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
 
     const double a0 = prm.a[0];
     const u64 lowerm = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    /* the history bit a child writes is its path's low bit, the branch bit;
+     * kept opaque so the update below stays one bit-field insert per word */
+    u64 brm = ((mask & 1u) && (lane & 1)) ? ~0ull : 0ull;
+    __asm__ volatile("" : "+v"(brm));
     /* global address space: these accesses count against vmcnt only, so the
      * prefetched block stays in flight across the step's LDS waits */
     const GPTR(double) in = (const GPTR(double))fl((long long)jb->in);
     GPTR(u64) out = (GPTR(u64))fl((long long)jb->out);
-    const int64_t n_in = fl(jb->n_in), keep_from = fl(jb->keep_from);
+    /* wave-uniform: the step loop's tests stay scalar */
+    const int64_t n_in = uni_ll(jb->n_in), keep_from = uni_ll(jb->keep_from);
     u64 acc = 0;
     int nacc = 0;
     int64_t kept = 0, slow_steps = 0;
@@ -536,7 +541,10 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
         const double cost = pcost + e * e;
         const uint32_t path = ((ppath << 1) | (uint32_t)br) & mask;
         const int cnext = pnext;
-        if (ballot(live && !__builtin_isfinite(cost))) err = 1;
+        /* masks of live lanes are formed as ballot(test) & ALL: a ballot of a
+         * plain compare is the compare's own lane mask */
+        const u64 ALL = (m == 64) ? ~0ull : ((1ull << m) - 1ull);
+        if (ballot(!__builtin_isfinite(cost)) & ALL) err = 1;
 
         const unsigned chi = (unsigned)((u64)bits(cost) >> 32);
         if (live) {
@@ -547,7 +555,6 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
         wsync();
 
         /* ---- 2. prune ---- */
-        const u64 ALL = (m == 64) ? ~0ull : ((1ull << m) - 1ull);
         /* LT bit j: child j has a strictly smaller metric (bits >= m are
          * don't-care; every use masks them off).  First on the high words
          * (metrics are >= 0; a set sign bit takes the full compare); equal
@@ -565,7 +572,7 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
             return t;
         };
         {
-            const bool neg = ballot(live && (chi >> 31)) != 0;
+            const bool neg = (ballot((int)chi < 0) & ALL) != 0;
             if (!neg) {
                 LT = (m <= 16) ? lt_sweep_hi<16>(s.khi, chi)
                    : (m <= 32) ? lt_sweep_hi<32>(s.khi, chi) : lt_sweep_hi<64>(s.khi, chi);
@@ -593,10 +600,10 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
         }
         bool slow = !groups_ok || tie;
 
-        const u64 N1 = ballot(live && cnext);
+        const u64 N1 = ballot(cnext != 0) & ALL;
         int nsurv = 0, myrank = 255;
         if (!slow) {
-            const int min0 = ff1(ballot(live && (LT & ALL) == 0));
+            const int min0 = ff1(ballot((LT & ALL) == 0) & ALL);
             const int v1 = __popcll(N1);
             const int maj = (v1 > m - v1) ? 1 : 0;
             int mn = min0;
@@ -638,7 +645,7 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
              * K passing ones before it), and groups of three or more, go to
              * the fixed point below. */
             bool done = false;
-            if (ballot(cand && __popcll(G) > 2) == 0) {
+            if ((ballot(__popcll(G) > 2) & CAND) == 0) {
                 const u64 pm = G & ~(1ull << lane);
                 const int pl = pm ? ff1(pm) : 0;
                 const bool pb = pm && ((LT >> pl) & 1);         /* partner precedes: dropped */
@@ -743,7 +750,7 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
 
         /* ---- 3. commit ---- */
         const bool surv = live && myrank < nsurv;
-        const u64 B0 = ballot(surv && myrank == 0);
+        const u64 B0 = ballot(myrank == 0) & (nsurv > 0 ? ALL : 0ull);
         const int r0 = ff1(B0);
         const int outbit = (int)((N1 >> r0) & 1);
         const double minc = rl_d(cost, r0);
@@ -762,7 +769,8 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
 #pragma unroll
             for (int w = 0; w < HW; w++) {
                 u64 hv = phist[w];
-                if (w == pw) hv = (path & 1u) ? (hv | pbit) : (hv & ~pbit);
+                /* one word: pos < L <= 64 is always in it */
+                if (HW == 1 || w == pw) hv = (hv & ~pbit) | (brm & pbit);
                 rhist[w] = hv;
             }
         }
