@@ -455,10 +455,16 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
     __asm__ volatile("" : "+v"(brm));
     /* global address space: these accesses count against vmcnt only, so the
      * prefetched block stays in flight across the step's LDS waits */
-    const GPTR(double) in = (const GPTR(double))fl((long long)jb->in);
-    GPTR(u64) out = (GPTR(u64))fl((long long)jb->out);
+    const GPTR(double) in = (const GPTR(double))uni_ll((long long)jb->in);
+    GPTR(u64) out = (GPTR(u64))uni_ll((long long)jb->out);
     /* wave-uniform: the step loop's tests stay scalar */
     const int64_t n_in = uni_ll(jb->n_in), keep_from = uni_ll(jb->keep_from);
+    /* src/trellis.c:1013-1040: the first L - pending steps only fill the
+     * traceback (pending counts up to L); after that the output of step t
+     * is kept from keep_from on.  So step t keeps its output exactly when
+     * t >= t_keep, and pending ends at min(L, pending + n_in). */
+    const int64_t t_fill = pending < L ? (int64_t)(L - pending) : 0;
+    const int64_t t_keep = t_fill > keep_from ? t_fill : keep_from;
     u64 acc = 0;
     int nacc = 0;
     int64_t kept = 0, slow_steps = 0;
@@ -785,9 +791,7 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
         idx = nb;
 
         /* output bookkeeping (src/trellis.c:1013-1040) */
-        if (pending < L) {
-            pending++;
-        } else if (t >= keep_from) {
+        if (t >= t_keep) {
             acc |= (u64)outbit << nacc;
             if (++nacc == 64) {
                 if (lane == 0) out[kept >> 6] = acc;
@@ -797,6 +801,7 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
             }
         }
     }
+    if (pending < L) pending = (int)((int64_t)pending + n_in < (int64_t)L ? (int64_t)pending + n_in : (int64_t)L);
     if (nacc) {
         if (lane == 0) out[kept >> 6] = acc;
         kept += nacc;
