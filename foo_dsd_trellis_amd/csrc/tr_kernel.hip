@@ -391,7 +391,7 @@ __device__ int wide_prune(TrLds<P, HW> &s, int nb, int m, int K, const tr_ctx *s
 /* One job on one wave; s is the wave's own LDS region.  Nothing here
  * synchronises beyond the wave, so several waves of a workgroup may run
  * different jobs (and different numbers of them). */
-template <int P, int HW, bool F8 = false, bool UJ = false>
+template <int P, int HW, bool F8 = false, bool UJ = false, int LIM = 2>
 __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr_ctx *ctxs,
                                             TrLds<P, HW> &s, const int lane)
 {
@@ -534,7 +534,9 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
             v = f ? vf : v;
         }
         d[0] = d[0] + (br ? -1.0 : 1.0);
-        if (prm.state_limit > 0.0) {
+        /* LIM: 0 no state limit, 1 a state limit, 2 read from prm (the
+         * kernels that dispatch on it get a step loop without the test) */
+        if (LIM == 1 || (LIM == 2 && prm.state_limit > 0.0)) {
             /* src/trellis.c:179-187, as selects (no per-lane branches) */
             const double lim = prm.state_limit, nlim = -prm.state_limit;
 #pragma unroll
@@ -863,7 +865,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TrWaves<HW>:
     __shared__ TrLds<P, HW> s;
     tr_job *jb = &jobs[blockIdx.x];
     if (jb->skip) return;
-    tr_job_body<P, HW>(prm, jb, ctxs, s, threadIdx.x);
+    if (prm.state_limit > 0.0) tr_job_body<P, HW, false, false, 1>(prm, jb, ctxs, s, threadIdx.x);
+    else tr_job_body<P, HW, false, false, 0>(prm, jb, ctxs, s, threadIdx.x);
 }
 
 /* The same with the order-8 FMA filter of the reference's AVX2 build. */
@@ -967,7 +970,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TrWaves<HW>:
         reinterpret_cast<uint64_t *>(&jd)[lane] = __hip_atomic_load(
             reinterpret_cast<const uint64_t *>(&hjobs[j]) + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     wsync();
-    if (!uni(jd.skip)) tr_job_body<P, HW, false, true>(prm, &jd, ctxs, s, lane);
+    if (!uni(jd.skip)) {
+        if (prm.state_limit > 0.0) tr_job_body<P, HW, false, true, 1>(prm, &jd, ctxs, s, lane);
+        else tr_job_body<P, HW, false, true, 0>(prm, &jd, ctxs, s, lane);
+    }
     wsync();
     if (lane < NW) reinterpret_cast<uint64_t *>(&jobs[j])[lane] = reinterpret_cast<const uint64_t *>(&jd)[lane];
     if (lane == 0) __hip_atomic_fetch_add(&dc->done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
