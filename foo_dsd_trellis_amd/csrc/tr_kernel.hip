@@ -41,6 +41,8 @@ __device__ __forceinline__ double rl_d(double v, int l)
 __device__ __forceinline__ int rl_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 __device__ __forceinline__ u64 ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 __device__ __forceinline__ int ff1(u64 m) { return (int)__builtin_ctzll(m); }
+/* this lane's bit of a wave-uniform mask, used as the lane predicate itself */
+__device__ __forceinline__ bool lane_in(u64 m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
 __device__ __forceinline__ long long bits(double d) { return __double_as_longlong(d); }
 
 /* count of set bits of `m` below this lane */
@@ -617,7 +619,7 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
             int mn = min0;
             if ((int)((N1 >> min0) & 1) != maj) {
                 const u64 MJ = maj ? N1 : (ALL & ~N1);
-                const u64 bmm = ballot(live && ((MJ >> lane) & 1) && (LT & MJ) == 0);
+                const u64 bmm = ballot((LT & MJ) == 0) & MJ;
                 if (bmm) {
                     const int bm = ff1(bmm);
                     if (rl_d(cost, bm) < rl_d(cost, min0) * 1.1) mn = bm;
@@ -625,7 +627,7 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
             }
             const int M = (int)((N1 >> mn) & 1);
             const u64 CAND = M ? N1 : (ALL & ~N1);
-            const bool cand = (CAND >> lane) & 1;
+            const bool cand = lane_in(CAND);
 
             /* same-path groups among candidates (paths are < 256 here) */
             const int gi = cand ? (int)path : 256 + lane;
@@ -654,25 +656,27 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
              * the fixed point below. */
             bool done = false;
             if ((ballot(__popcll(G) > 2) & CAND) == 0) {
+                /* pm: the partner's lane bit (a group holds at most two), 0
+                 * without one; every per-lane test below is a plain compare
+                 * whose ballot is its lane mask, and the rest is scalar */
                 const u64 pm = G & ~(1ull << lane);
-                const int pl = pm ? ff1(pm) : 0;
-                const bool pb = pm && ((LT >> pl) & 1);         /* partner precedes: dropped */
-                const u64 DROP = ballot(pb);
-                const u64 E = ballot(pb && pl < lane);
+                const u64 DROP = ballot((LT & pm) != 0);         /* partner precedes: dropped */
+                const u64 EARLY = ballot(pm != 0) & ballot(pm <= lowerm);   /* partner has the lower index */
+                const u64 E = DROP & EARLY;
                 /* this lane takes its earlier, worse partner off the list */
-                const u64 LATE = wave_excl_or((pm && pl < lane && !pb) ? (1ull << pl) : 0ull);
-                const bool ps = cand && __popcll(LT & CAND & lowerm & ~(E | LATE)) < K;
-                const u64 PS = ballot(ps);
+                const u64 LATE = wave_excl_or(lane_in(EARLY & ~DROP) ? pm : 0ull);
+                const u64 X = CAND & ~E;
+                const u64 PS = ballot(__popcll(LT & lowerm & ~LATE & X) < K) & CAND;
                 bool decline = false;
                 if (K == TR_MAXK && nb == 1)
-                    decline = ballot(ps && (path & 127u) == 0 && __popcll(PS & lowerm) > K) != 0;
+                    decline = (PS & ballot((path & 127u) == 0) & ballot(mbcnt(PS) > K)) != 0;
                 if (!decline) {
-                    const bool rg = ps && !(pm && pl < lane && ((PS >> pl) & 1));
-                    const u64 tw = ballot(rg) & G2;
-                    if (rg) s.link[nb][lane] = (tw & lowerm) ? (int8_t)ff1(tw) : (int8_t)-1;
+                    const u64 REG = PS & ~(EARLY & ballot((PS & pm) != 0));
+                    const u64 tw = REG & G2;
+                    if (lane_in(REG)) s.link[nb][lane] = (tw & lowerm) ? (int8_t)ff1(tw) : (int8_t)-1;
                     const u64 F = CAND & ~DROP;
                     const int rk = __popcll(LT & F);
-                    myrank = (((F >> lane) & 1) && rk < K) ? rk : 255;
+                    myrank = lane_in(F & ballot(rk < K)) ? rk : 255;
                     const int nf = __popcll(F);
                     nsurv = nf < K ? nf : K;
                     done = true;
@@ -742,7 +746,7 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
                     if (reg) s.link[nb][lane] = (tw & lowerm) ? (int8_t)ff1(tw) : (int8_t)-1;
                     const u64 F = CAND & ~DS & ~KM;
                     const int rk = __popcll(LT & F);
-                    myrank = (((F >> lane) & 1) && rk < K) ? rk : 255;
+                    myrank = lane_in(F & ballot(rk < K)) ? rk : 255;
                     const int nf = __popcll(F);
                     nsurv = nf < K ? nf : K;
                 }
