@@ -226,11 +226,13 @@ struct TrLds {
     int8_t link[2][64];
     uint8_t rank[64];
     int8_t head[128];
-    int8_t src[TR_MAXK];           /* lane holding the survivor of each rank */
+    uint8_t src[TR_MAXK];          /* lane holding the survivor of each rank, x 4
+                                      (the bpermute byte address) */
 };
 
 /* value of another lane's register (all lanes active) */
-__device__ __forceinline__ int bp_i(int v, int from) { return __builtin_amdgcn_ds_bpermute(from << 2, v); }
+/* from: byte address of the source lane (lane * 4) */
+__device__ __forceinline__ int bp_i(int v, int from) { return __builtin_amdgcn_ds_bpermute(from, v); }
 __device__ __forceinline__ double bp_d(double v, int from)
 {
     const u64 b = (u64)__double_as_longlong(v);
@@ -434,7 +436,7 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
             rnext = c0->next[lane];
             myr = lane;
         }
-        if (lane < TR_MAXK) s.src[lane] = (int8_t)lane;
+        if (lane < TR_MAXK) s.src[lane] = (uint8_t)(lane << 2);
         for (int b = 0; b < 2; b++) {
             int8_t l = c0->link[b][lane];
             if (fi(jb->copy)) {
@@ -467,6 +469,9 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
      * t >= t_keep, and pending ends at min(L, pending + n_in). */
     const int64_t t_fill = pending < L ? (int64_t)(L - pending) : 0;
     const int64_t t_keep = t_fill > keep_from ? t_fill : keep_from;
+    /* steps left before the first kept one (counted down: a 64-bit zero
+     * test is scalar, an ordered 64-bit compare is not) */
+    int64_t kc = t_keep > 0 ? t_keep : 0;
     u64 acc = 0;
     int nacc = 0;
     int64_t kept = 0, slow_steps = 0;
@@ -492,7 +497,9 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
         if (next_pos == L) next_pos = 0;
 
         /* ---- 1. expand ---- */
-        const int from = s.src[k] & 63;        /* lane holding parent k (k >= n: unused) */
+        /* byte address (lane * 4) of the lane holding parent k; entries
+         * k >= n are stale lane addresses, read and unused */
+        const int from = s.src[k];
         double sp[P], d[P];
 #pragma unroll
         for (int q = 0; q < P; q++) sp[q] = bp_d(rst[q], from);
@@ -574,10 +581,10 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
         bool tie = false;
         /* equal keys: two children with one strict rank */
         auto has_tie = [&](u64 lt) {
-            const int r = __popcll(lt & ALL);
+            const int r = __popcll(lt & ALL) & 63;   /* a live lane's is <= m - 1 */
             if (live) s.rank[r] = (uint8_t)lane;
             wsync();
-            const bool t = ballot(live && s.rank[r] != lane) != 0;
+            const bool t = (ballot(s.rank[r] != lane) & ALL) != 0;
             wsync();
             return t;
         };
@@ -617,7 +624,7 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
             const int v1 = __popcll(N1);
             const int maj = (v1 > m - v1) ? 1 : 0;
             int mn = min0;
-            if ((int)((N1 >> min0) & 1) != maj) {
+            if (((N1 >> min0) ^ (u64)maj) & 1) {
                 const u64 MJ = maj ? N1 : (ALL & ~N1);
                 const u64 bmm = ballot((LT & MJ) == 0) & MJ;
                 if (bmm) {
@@ -635,9 +642,10 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
             wsync();
             const u64 G = s.gtab[gi] & (cand ? ~0ull : 0ull);
             /* the group of the other path of this hash bucket (path ^ 128) */
-            const u64 G2 = cand ? s.gtab[path ^ 128u] : 0ull;
+            const u64 G2 = cand ? s.gtab[gi ^ 128] : 0ull;
             wsync();
-            s.gtab[gi] = vzero64();
+            /* every member takes its own bit out again: the entry is 0 */
+            atomicXor(&s.gtab[gi], 1ull << lane);
 
             /* Closed form where every same-path group holds at most two
              * candidates (two children share a path only when their
@@ -788,7 +796,7 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
         }
         if (surv) {
             s.ncost[nb][lane] = nc;
-            s.src[myrank] = (int8_t)lane;
+            s.src[myrank] = (uint8_t)(lane << 2);
         }
         myr = surv ? myrank : 255;
         wsync();
@@ -797,7 +805,9 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
         idx = nb;
 
         /* output bookkeeping (src/trellis.c:1013-1040) */
-        if (t >= t_keep) {
+        if (kc != 0) {
+            kc--;
+        } else {
             acc |= (u64)outbit << nacc;
             if (++nacc == 64) {
                 if (lane == 0) out[kept >> 6] = acc;
