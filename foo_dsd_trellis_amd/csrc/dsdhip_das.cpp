@@ -248,7 +248,8 @@ static dsdhip_das_t *das_create(const double *a, const double *g, int order, int
     d->est_device = 0;
     d->batch_launch = 0;
     d->feed_limit_s = 20.0;
-    d->feed_waves_cu = 8;
+    d->feed_waves_cu = 6;         /* 30 s stereo to DSD256, 65,536-sample segments: time from the last
+                                     estimate to the end, 4 waves per CU 203 ms, 5: 128, 6: 138, 8: 145 */
     d->last_est_ms = 0.0;
     d->last_est_samples = 0;
     d->last_est_redo = 0;

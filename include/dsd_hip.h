@@ -346,9 +346,10 @@ int dsdhip_das_batch_launch(dsdhip_das_t *d, int on);
  * for its segment; a wave that waits longer leaves and the run returns -1.
  * No reference counterpart. */
 int dsdhip_das_feed_limit(dsdhip_das_t *d, double seconds);
-/* Resident waves per CU of the host-fed launch (default 8, i.e. two per
- * SIMD; 0: as many as the kernel's registers allow).  No reference
- * counterpart. */
+/* Resident waves per CU of the host-fed launch (default 6; 0: as many as
+ * the kernel's registers allow).  Fewer waves per SIMD run the last
+ * published segments faster; too few leave published segments waiting for
+ * a slot.  No reference counterpart. */
 int dsdhip_das_feed_waves(dsdhip_das_t *d, int waves_per_cu);
 /* Host-side timeline of the last seg0-first run, ms from the run's start:
  * seg0 done, estimate threads started, last segment published, estimate
