@@ -9,9 +9,11 @@
  *   2. prune    : a wave-parallel formulation of the reference's sequential
  *                 candidate sort (src/trellis.c:621-731): order masks by
  *                 (metric, index) from an LDS broadcast sweep, majority/
- *                 traceback filter, then a fixed-point resolution of the
- *                 early skip and of the duplicate-path rules, and a final
- *                 rank; steps that fall outside what the parallel form
+ *                 traceback filter, then the early skip and the
+ *                 duplicate-path rules in closed form on lane masks where
+ *                 every same-path group holds at most two candidates (a
+ *                 fixed-point resolution otherwise), and a final rank;
+ *                 steps that fall outside what the parallel form
  *                 decides (a bucket-0 lookup after the full-list write past
  *                 act[31] at K=32, paths wider than 8 bits) step the
  *                 sequential sort literally, child by child, with the list
@@ -69,15 +71,6 @@ __device__ __forceinline__ u64 wave_excl_or(u64 v)
 {
     const u64 inc = ((u64)scan_or32((unsigned)(v >> 32)) << 32) | scan_or32((unsigned)v);
     return inc & ~v;
-}
-
-/* A 64-bit zero materialised where it is used (the register allocator
- * otherwise keeps one zero pair alive across the step and spills it). */
-__device__ __forceinline__ u64 vzero64()
-{
-    unsigned lo, hi;
-    __asm__ volatile("v_mov_b32 %0, 0\n\tv_mov_b32 %1, 0" : "=v"(lo), "=v"(hi));
-    return ((u64)hi << 32) | lo;
 }
 
 /* The workgroup is one wave: its LDS instructions execute in issue order,
