@@ -572,6 +572,7 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
          * sweep runs. */
         u64 LT;
         bool tie = false;
+        const u64 N1 = ballot(cnext != 0) & ALL;
         /* equal keys: two children with one strict rank */
         auto has_tie = [&](u64 lt) {
             const int r = __popcll(lt & ALL) & 63;   /* a live lane's is <= m - 1 */
@@ -581,12 +582,58 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
             wsync();
             return t;
         };
+        /* candidates: first strict minimum, majority of next with the best
+         * majority member's 1.1 override, then the next filter */
+        auto pick_cand = [&]() -> u64 {
+            const int min0 = ff1(ballot((LT & ALL) == 0) & ALL);
+            const int v1 = __popcll(N1);
+            const int maj = (v1 > m - v1) ? 1 : 0;
+            int mn = min0;
+            if (((N1 >> min0) ^ (u64)maj) & 1) {
+                const u64 MJ = maj ? N1 : (ALL & ~N1);
+                const u64 bmm = ballot((LT & MJ) == 0) & MJ;
+                if (bmm) {
+                    const int bm = ff1(bmm);
+                    if (rl_d(cost, bm) < rl_d(cost, min0) * 1.1) mn = bm;
+                }
+            }
+            return ((N1 >> mn) & 1) ? N1 : (ALL & ~N1);
+        };
+        /* same-path groups among candidates (paths are < 256 here) in one
+         * LDS exchange, which also carries the tie test when rank_too: G,
+         * the lane's group, G2 the group of the other path of its hash
+         * bucket (path ^ 128) */
+        u64 CAND = 0, G = 0, G2 = 0;
+        auto group_pass = [&](bool rank_too) -> bool {
+            const bool cand = lane_in(CAND);
+            const int gi = cand ? (int)path : 256 + lane;
+            const int r = __popcll(LT & ALL) & 63;
+            if (rank_too && live) s.rank[r] = (uint8_t)lane;
+            atomicOr(&s.gtab[gi], 1ull << lane);
+            wsync();
+            const bool t = rank_too && (ballot(s.rank[r] != lane) & ALL) != 0;
+            G = s.gtab[gi] & (cand ? ~0ull : 0ull);
+            G2 = cand ? s.gtab[gi ^ 128] : 0ull;
+            wsync();
+            /* every member takes its own bit out again: the entry is 0 */
+            atomicXor(&s.gtab[gi], 1ull << lane);
+            return t;
+        };
+        bool grouped = false;
         {
             const bool neg = (ballot((int)chi < 0) & ALL) != 0;
             if (!neg) {
                 LT = (m <= 16) ? lt_sweep_hi<16>(s.khi, chi)
                    : (m <= 32) ? lt_sweep_hi<32>(s.khi, chi) : lt_sweep_hi<64>(s.khi, chi);
-                tie = has_tie(LT);
+                if (groups_ok) {
+                    /* candidates and groups from these masks, the tie test in
+                     * the same exchange; with a tie all of it is redone below */
+                    CAND = pick_cand();
+                    tie = group_pass(true);
+                    grouped = !tie;
+                } else {
+                    tie = has_tie(LT);
+                }
             }
             if (neg || tie) {
                 const double *keys = s.ncost[nb];
@@ -610,35 +657,13 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
         }
         bool slow = !groups_ok || tie;
 
-        const u64 N1 = ballot(cnext != 0) & ALL;
         int nsurv = 0, myrank = 255;
         if (!slow) {
-            const int min0 = ff1(ballot((LT & ALL) == 0) & ALL);
-            const int v1 = __popcll(N1);
-            const int maj = (v1 > m - v1) ? 1 : 0;
-            int mn = min0;
-            if (((N1 >> min0) ^ (u64)maj) & 1) {
-                const u64 MJ = maj ? N1 : (ALL & ~N1);
-                const u64 bmm = ballot((LT & MJ) == 0) & MJ;
-                if (bmm) {
-                    const int bm = ff1(bmm);
-                    if (rl_d(cost, bm) < rl_d(cost, min0) * 1.1) mn = bm;
-                }
+            if (!grouped) {
+                CAND = pick_cand();
+                group_pass(false);
             }
-            const int M = (int)((N1 >> mn) & 1);
-            const u64 CAND = M ? N1 : (ALL & ~N1);
             const bool cand = lane_in(CAND);
-
-            /* same-path groups among candidates (paths are < 256 here) */
-            const int gi = cand ? (int)path : 256 + lane;
-            atomicOr(&s.gtab[gi], 1ull << lane);
-            wsync();
-            const u64 G = s.gtab[gi] & (cand ? ~0ull : 0ull);
-            /* the group of the other path of this hash bucket (path ^ 128) */
-            const u64 G2 = cand ? s.gtab[gi ^ 128] : 0ull;
-            wsync();
-            /* every member takes its own bit out again: the entry is 0 */
-            atomicXor(&s.gtab[gi], 1ull << lane);
 
             /* Closed form where every same-path group holds at most two
              * candidates (two children share a path only when their
