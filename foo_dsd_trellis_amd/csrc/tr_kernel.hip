@@ -997,6 +997,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TrWaves<HW>:
     int st = 0;
     if (lane == 0) st = feed_wait(hc, dc, j, poll, limit, max_iter);
     if (__builtin_amdgcn_readfirstlane(st)) return;
+    /* the jobs published last set the launch's end: they take issue
+     * priority over the earlier ones that share their SIMD */
+    const int nj = (int)gridDim.x;
+    if (j >= nj - nj / 32) __builtin_amdgcn_s_setprio(3);
+    else if (j >= nj - nj / 8) __builtin_amdgcn_s_setprio(2);
+    else if (j >= nj - nj / 4) __builtin_amdgcn_s_setprio(1);
     /* lane w copies word w of the descriptor into LDS */
     if (lane < NW)
         reinterpret_cast<uint64_t *>(&jd)[lane] = __hip_atomic_load(
