@@ -2,7 +2,7 @@
 2266): seg0 on the persistent context, seg1 its exact continuation, every
 later segment seeded by the 16-level estimate from seg1's start (host
 threads, segments launched as their estimates arrive).  Bit-exact against
-the oracle at the C2 shape (DSD256, K=32, L=64, 65,536-sample segments),
+the oracle at the headline shape (DSD256, K=32, L=64, 65,536-sample segments),
 with the output count the reference's plugin checks (F, or F - L on the
 first chunk, :2733-2747) and the host-buffer output's tail."""
 import ctypes as C
