@@ -63,6 +63,7 @@ _SIGS = {
     "dsdhip_das_feed_limit": ([_vp, C.c_double], C.c_int),
     "dsdhip_das_feed_waves": ([_vp, C.c_int], C.c_int),
     "dsdhip_das_last_timeline": ([_vp, C.POINTER(C.c_double)], C.c_int),
+    "dsdhip_stream_last_timeline": ([_vp, C.POINTER(C.c_double)], C.c_int),
     "dsdhip_das_last_estimate": ([_vp, _dp, _i64p, _i64p, _ip], C.c_int),
     "dsdhip_das_wave_only": ([_vp, C.c_int], C.c_int),
     "gpu_hip_fir_lowpass_setup": ([_vp, _fp, C.c_int], C.c_int),

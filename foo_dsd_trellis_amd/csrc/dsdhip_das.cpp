@@ -95,13 +95,22 @@ struct DasScratch {
     hipEvent_t ev_go = 0, ev_t0 = 0, ev_t1 = 0;
     double *h_est = nullptr;
     size_t h_est_cap = 0;
+    /* the staging copies run on their own stream, behind ev_in (the input
+     * is ready) only, so they overlap segment 0; per estimate chain: its
+     * offset in h_est and its first block event (npos: not staged) */
+    hipStream_t cs = 0;
+    hipEvent_t ev_in = 0;
+    int staged = 0;
+    std::vector<size_t> st_base, st_ev;
     int pipe_ok = 0;
     int init_pipe()
     {
         if (pipe_ok) return 0;
         for (int i = 0; i < NBS; i++)
             if (hipStreamCreateWithFlags(&bs[i], hipStreamNonBlocking) != hipSuccess) return -1;
+        if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return -1;
         if (hipEventCreateWithFlags(&ev_go, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&ev_in, hipEventDisableTiming) != hipSuccess ||
             hipEventCreate(&ev_t0) != hipSuccess || hipEventCreate(&ev_t1) != hipSuccess)
             return -1;
         pipe_ok = 1;
@@ -132,6 +141,8 @@ struct DasScratch {
         for (hipEvent_t e : blk) hipEventDestroy(e);
         for (hipStream_t x : bs)
             if (x) hipStreamDestroy(x);
+        if (cs) hipStreamDestroy(cs);
+        if (ev_in) hipEventDestroy(ev_in);
         if (ev_go) hipEventDestroy(ev_go);
         if (ev_t0) hipEventDestroy(ev_t0);
         if (ev_t1) hipEventDestroy(ev_t1);
@@ -641,6 +652,64 @@ static double ms_since(const dsdhip_das_t *d)
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - d->t_run0).count();
 }
 
+/* Queues the copies of the estimate input (each channel from seg1's start
+ * to the end, for tracks of more than two segments) into pinned host memory
+ * on the copy stream, behind ev_in only: block i of every chain before
+ * block i + 1 of any, each followed by its event.  Called once segment 0's
+ * launch is queued, so the copies run beside segment 0 and the estimator
+ * chains, which start when segment 0 ends, find their input across. */
+static int estimate_stage(dsdhip_das_t *d, const std::vector<TrackRun> &R)
+{
+    DasScratch &sc = d->sc;
+    const int nch = d->nch, T = d->ntracks;
+    sc.staged = 0;
+    sc.st_base.clear();
+    sc.st_ev.clear();
+    std::vector<const double *> dx;
+    std::vector<size_t> cnt;
+    size_t staged = 0, nblk = 0;
+    for (int t = 0; t < T; t++) {
+        const TrackRun &r = R[(size_t)t];
+        if (r.S <= 2) continue;
+        const int64_t es = (int64_t)r.st[1];
+        for (int ch = 0; ch < nch; ch++) {
+            if (r.hin) {
+                sc.st_base.push_back((size_t)-1);
+                sc.st_ev.push_back(0);
+                dx.push_back(nullptr);
+                cnt.push_back(0);
+                continue;
+            }
+            const size_t n = r.F - (size_t)es;
+            sc.st_base.push_back(staged);
+            sc.st_ev.push_back(nblk);
+            dx.push_back(r.in + (int64_t)ch * r.in_stride + es);
+            cnt.push_back(n);
+            staged += n;
+            nblk += (n + EST_BLOCK - 1) / EST_BLOCK;
+        }
+    }
+    if (staged == 0) return 0;
+    if (sc.ensure_host(staged) != 0 || DasScratch::grow_events(sc.blk, nblk) != 0 ||
+        hipStreamWaitEvent(sc.cs, sc.ev_in, 0) != hipSuccess)
+        return -1;
+    for (size_t b = 0;; b++) {
+        bool any = false;
+        for (size_t i = 0; i < dx.size(); i++) {
+            if (!dx[i] || b * EST_BLOCK >= cnt[i]) continue;
+            any = true;
+            const size_t n = std::min(EST_BLOCK, cnt[i] - b * EST_BLOCK);
+            if (hipMemcpyAsync(sc.h_est + sc.st_base[i] + b * EST_BLOCK, dx[i] + b * EST_BLOCK,
+                               sizeof(double) * n, hipMemcpyDeviceToHost, sc.cs) != hipSuccess ||
+                hipEventRecord(sc.blk[sc.st_ev[i] + b], sc.cs) != hipSuccess)
+                return -1;
+        }
+        if (!any) break;
+    }
+    sc.staged = 1;
+    return 0;
+}
+
 static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned long long *segbits)
 {
     const int nch = d->nch, T = d->ntracks;
@@ -676,6 +745,9 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
             if (dsdhip_trellis_best_state(tr, ix_persist(d, t * nch + ch), c.init) != 0) return -1;
         }
     }
+    /* the staging copies, queued beside segment 0 (estimate_stage) */
+    const bool prestaged = sc.staged && sc.st_base.size() == nchains;
+    sc.staged = 0;
     d->last_batches = 0;
     d->last_est_ms = 0.0;
     d->last_est_samples = 0;
@@ -683,7 +755,9 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
     d->last_tl[0] = ms_since(d);      /* seg0 done: its best states were read */
     if (total == 0) return 0;
     if (dsdhip_trellis_table_begin(tr, total) != 0) return -1;
-    if (staged && (sc.ensure_host(staged) != 0 || DasScratch::grow_events(sc.blk, nblk) != 0)) return -1;
+    if (!prestaged && staged &&
+        (sc.ensure_host(staged) != 0 || DasScratch::grow_events(sc.blk, nblk) != 0))
+        return -1;
     if (DasScratch::grow_events(sc.bev, (size_t)total + 1) != 0) return -1;
     ent.reserve((size_t)total);
     auto fill = [&](int t, int ch, int seg, const double *seed) -> int {
@@ -702,10 +776,17 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
     if (timing) hipEventRecord(sc.ev_t0, d->stream);
     for (int i = 0; i < DasScratch::NBS; i++)
         if (hipStreamWaitEvent(sc.bs[i], sc.ev_go, 0) != hipSuccess) return -1;
-    /* stage the estimate input: every copy is queued now, block i of every
-     * chain before block i + 1 of any, each followed by its event */
+    /* stage the estimate input if segment 0's call did not: every copy is
+     * queued now, block i of every chain before block i + 1 of any, each
+     * followed by its event */
     int rc = 0;
-    {
+    if (prestaged) {
+        for (size_t i = 0; i < nchains; i++) {
+            if (chains[i].hx || sc.st_base[i] == (size_t)-1) continue;
+            chains[i].hx = sc.h_est + sc.st_base[i];
+            chains[i].blk = &sc.blk[sc.st_ev[i]];
+        }
+    } else {
         size_t off = 0, ei = 0;
         std::vector<size_t> base(nchains, 0), first_ev(nchains, 0);
         for (size_t i = 0; i < nchains; i++) {
@@ -732,10 +813,11 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
             if (!any) break;
         }
     }
-    /* host-fed launch: on the DAS stream behind every staging copy, since
-     * its waves wait for estimates that need those copies (a launch queued
-     * ahead of them, on any stream sharing their hardware queue, would
-     * never see its entries published) */
+    /* host-fed launch: queued after every staging copy (on the DAS stream
+     * behind them, or on the copy stream before it), since its waves wait
+     * for estimates that need those copies (a launch queued ahead of them,
+     * on any stream sharing their hardware queue, would never see its
+     * entries published) */
     int fed = 1;
     if (rc == 0 && !d->batch_launch) {
         fed = dsdhip_trellis_feed_begin(tr, d->stream, d->feed_limit_s, d->feed_waves_cu);
@@ -950,7 +1032,7 @@ static int run_tracks(dsdhip_das_t *d, const double *const *d_in, const int64_t 
         const double nan = nan_val();
         for (int q = 0; q < 10; q++) seeds.push_back(seed ? seed[q] : nan);
     };
-    auto run = [&]() -> int {
+    auto run = [&](int (*queued)(void *) = nullptr, void *qarg = nullptr) -> int {
         const int nj = (int)jl.size();
         if (nj == 0) return 0;
         std::vector<int> ctx((size_t)nj), cs((size_t)nj);
@@ -969,9 +1051,9 @@ static int run_tracks(dsdhip_das_t *d, const double *const *d_in, const int64_t 
             kf[(size_t)j] = r.keep[(size_t)q.seg];
         }
         int err = 0;
-        const int rc = dsdhip_trellis_run_jobs(tr, nj, ctx.data(), cs.data(), pin.data(), ni.data(),
-                                               kf.data(), pout.data(), seeds.data(), kp.data(),
-                                               sl.data(), &err);
+        const int rc = dsdhip_trellis_run_jobs_then(tr, nj, ctx.data(), cs.data(), pin.data(), ni.data(),
+                                                    kf.data(), pout.data(), seeds.data(), kp.data(),
+                                                    sl.data(), &err, queued, qarg);
         for (int j = 0; j < nj; j++) {
             const JobRef &q = jl[(size_t)j];
             TrackRun &r = R[(size_t)q.t];
@@ -986,7 +1068,19 @@ static int run_tracks(dsdhip_das_t *d, const double *const *d_in, const int64_t 
     if (seg0_first) {
         for (int t = 0; t < T; t++)
             for (int ch = 0; ch < nch && R[(size_t)t].S > 0; ch++) add(t, ch, 0, -1, NULL);
-        if (run() != 0) return -1;
+        d->sc.staged = 0;
+        if (!d->est_device) {
+            /* the estimate input's copies run beside segment 0 */
+            struct StageArg { dsdhip_das_t *d; const std::vector<TrackRun> *R; } sa{d, &R};
+            if (d->sc.init_pipe() != 0 || hipEventRecord(d->sc.ev_in, d->stream) != hipSuccess) return -1;
+            if (run([](void *p) -> int {
+                    const StageArg *a = (const StageArg *)p;
+                    return estimate_stage(a->d, *a->R);
+                }, &sa) != 0)
+                return -1;
+        } else if (run() != 0) {
+            return -1;
+        }
         if (!d->est_device) {
             if (seg0_first_rest(d, R, segbits.p) != 0) return -1;
         } else {
@@ -1323,11 +1417,12 @@ int dsdhip_das_last_timeline(const dsdhip_das_t *d, double *ms6)
     return 0;
 }
 
-/* Resident waves per CU of the host-fed launch (default 8, two per SIMD;
- * 0: as many as the kernel's registers allow).  The segments arrive at the
- * estimator's pace, which a few waves per SIMD keep up with; fewer resident
- * waves keep the last segments on SIMDs of their own, so the step ends
- * about one segment's latency after the estimate. */
+/* Resident waves per CU of the host-fed launch (default 6, one or two per
+ * SIMD; 0: as many as the kernel's registers allow).  The segments arrive
+ * at the estimator's pace, which a few waves per SIMD keep up with; fewer
+ * resident waves leave the last segments fewer neighbours on their SIMD, so
+ * the step ends sooner after the last estimate, until published segments
+ * start to wait for a slot. */
 int dsdhip_das_feed_waves(dsdhip_das_t *d, int waves_per_cu)
 {
     if (!d || waves_per_cu < 0 || waves_per_cu > 64) return -1;

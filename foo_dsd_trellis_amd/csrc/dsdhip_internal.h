@@ -37,6 +37,13 @@ int dsdhip_trellis_timing_add(dsdhip_trellis_t *t, double ms, int64_t launches);
 int dsdhip_trellis_fill_job(const dsdhip_trellis_t *t, tr_job *jb, int ctx, int src, const double *d_in,
                             int64_t n_in, int64_t keep_from, uint64_t *d_out, const double *seed10);
 int dsdhip_trellis_set_snap(dsdhip_trellis_t *t, int i, int src);
+/* dsdhip_trellis_run_jobs that calls queued(arg) once the launch and its
+ * read-back are queued, before it waits for them (queued returns 0, or
+ * nonzero to fail the call after the wait) */
+int dsdhip_trellis_run_jobs_then(dsdhip_trellis_t *t, int njobs, const int *ctx, const int *copy_src,
+                                 const double *const *d_in, const int64_t *n_in, const int64_t *keep_from,
+                                 uint64_t *const *d_out, const double *seed, int64_t *kept,
+                                 int64_t *slow_steps, int *err_out, int (*queued)(void *), void *arg);
 /* 255-tap Kaiser low-pass of the stitch quality (src/dsp_plugin.c:2571-2604). */
 void dsdhip_make_qfir(double *q);
 /* Device scratch of the FIR chain for n input samples per channel, so later

@@ -19,6 +19,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <chrono>
+
 #include "dsdhip_internal.h"
 
 struct dsdhip_stream {
@@ -36,7 +38,16 @@ struct dsdhip_stream {
     double prev_gain, gain;
     size_t last_n;
     hipStream_t stream;
+    /* host-side timeline of the last chunk (ms from the call's start): DAS
+     * run started, then the DAS run's own six points */
+    std::chrono::steady_clock::time_point t_call;
+    double last_tl[7];
 };
+
+static double ms_since_call(const dsdhip_stream_t *s)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - s->t_call).count();
+}
 
 extern "C" {
 
@@ -200,7 +211,12 @@ static int finish(dsdhip_stream_t *s, size_t n, unsigned long long *d_bits, int6
         return -1;
     if (s->conv && dsdhip_conv_process_fifo_dev(s->conv, s->d_fir, st, n, s->d_sig, st) != 0) return -1;
     s->last_n = n;
-    return dsdhip_das_run_dev(s->das, s->d_sig, st, n, d_bits, out_words, out_count);
+    s->last_tl[0] = ms_since_call(s);
+    const int rc = dsdhip_das_run_dev(s->das, s->d_sig, st, n, d_bits, out_words, out_count);
+    double tl[6];
+    if (dsdhip_das_last_timeline(s->das, tl) == 0)
+        for (int i = 0; i < 6; i++) s->last_tl[i + 1] = tl[i] != 0.0 ? tl[i] + s->last_tl[0] : 0.0;
+    return rc;
 }
 
 /* frames PCM frames per channel (<= chunk_frames; with a room IR any length,
@@ -210,6 +226,8 @@ int dsdhip_stream_process_dev(dsdhip_stream_t *s, const float *d_pcm, int64_t pc
                               unsigned long long *d_bits, int64_t out_words, size_t *out_count)
 {
     if (!s || !d_pcm || !d_bits || frames > s->cfg.chunk_frames || s->cfg.input != DSDHIP_IN_PCM) return -1;
+    s->t_call = std::chrono::steady_clock::now();
+    for (int i = 0; i < 7; i++) s->last_tl[i] = 0.0;
     if (out_count) *out_count = 0;
     if (frames == 0) return 0;
     const size_t n = frames * (size_t)s->R;
@@ -237,6 +255,8 @@ int dsdhip_stream_process_dsd_dev(dsdhip_stream_t *s, const unsigned long long *
                                   size_t n, unsigned long long *d_bits, int64_t out_words, size_t *out_count)
 {
     if (!s || !d_in || !d_bits || n > s->cfg.chunk_frames || s->cfg.input != DSDHIP_IN_DSD) return -1;
+    s->t_call = std::chrono::steady_clock::now();
+    for (int i = 0; i < 7; i++) s->last_tl[i] = 0.0;
     if (out_count) *out_count = 0;
     if (n == 0) return 0;
     const size_t m = n * (size_t)s->R;
@@ -265,6 +285,17 @@ int dsdhip_stream_process_dsd_dev(dsdhip_stream_t *s, const unsigned long long *
 
 /* SDM input of the last chunk into dst [nch][dst_stride] doubles (host or
  * device memory); returns the samples per channel, or -1. */
+/* Host-side timeline of the last chunk, ms from the call's start: DAS run
+ * started, then the DAS run's six points (seg0 done, estimate started, last
+ * segment published, estimate joined, trellis read, run done; zero where
+ * the DAS did not pass them). */
+int dsdhip_stream_last_timeline(const dsdhip_stream_t *s, double *ms7)
+{
+    if (!s || !ms7) return -1;
+    for (int i = 0; i < 7; i++) ms7[i] = s->last_tl[i];
+    return 0;
+}
+
 int64_t dsdhip_stream_copy_signal(dsdhip_stream_t *s, double *dst, int64_t dst_stride)
 {
     if (!s || !dst || dst_stride < (int64_t)s->last_n) return -1;

@@ -562,6 +562,15 @@ int dsdhip_trellis_run_jobs(dsdhip_trellis_t *t, int njobs, const int *ctx, cons
                             const double *seed, int64_t *kept, int64_t *slow_steps,
                             int *err_out)
 {
+    return dsdhip_trellis_run_jobs_then(t, njobs, ctx, copy_src, d_in, n_in, keep_from, d_out, seed, kept,
+                                        slow_steps, err_out, nullptr, nullptr);
+}
+
+int dsdhip_trellis_run_jobs_then(dsdhip_trellis_t *t, int njobs, const int *ctx, const int *copy_src,
+                                 const double *const *d_in, const int64_t *n_in, const int64_t *keep_from,
+                                 uint64_t *const *d_out, const double *seed, int64_t *kept,
+                                 int64_t *slow_steps, int *err_out, int (*queued)(void *), void *arg)
+{
     if (!t || njobs < 0) return -1;
     if (njobs == 0) return 0;
     std::vector<tr_job> jobs((size_t)njobs);
@@ -581,7 +590,8 @@ int dsdhip_trellis_run_jobs(dsdhip_trellis_t *t, int njobs, const int *ctx, cons
     if (hipMemcpyAsync(jobs.data(), t->d_jobs, sizeof(tr_job) * (size_t)njobs,
                        hipMemcpyDeviceToHost, t->stream) != hipSuccess)
         return -1;
-    if (hipStreamSynchronize(t->stream) != hipSuccess) return -1;
+    const int qrc = queued ? queued(arg) : 0;
+    if (hipStreamSynchronize(t->stream) != hipSuccess || qrc != 0) return -1;
     if (t->timing) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, t->ev0, t->ev1) == hipSuccess) t->kernel_ms += ms;

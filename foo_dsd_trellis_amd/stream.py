@@ -31,6 +31,7 @@ def _sigs(L):
                                                 C.c_int64, C.POINTER(C.c_size_t)]
     L.dsdhip_stream_copy_signal.restype = C.c_int64
     L.dsdhip_stream_copy_signal.argtypes = [C.c_void_p, _dp, C.c_int64]
+    L.dsdhip_stream_last_timeline.argtypes = [C.c_void_p, _dp]
     L._stream_sigs = True
 
 
@@ -121,6 +122,14 @@ class Stream:
         if n < 0:
             raise _lib.DsdHipError("dsdhip_stream_copy_signal failed")
         return buf[:, :n]
+
+    def last_timeline(self):
+        """Host-side timeline of the last chunk, ms from the call's start:
+        DAS started, seg0 done, estimate started, last segment published,
+        estimate joined, trellis read, run done (0.0 where not passed)."""
+        t = (C.c_double * 7)()
+        _lib.check(self.lib.dsdhip_stream_last_timeline(self.h, t), "dsdhip_stream_last_timeline")
+        return list(t)
 
     def reset(self):
         _lib.check(self.lib.dsdhip_stream_reset(self.h), "dsdhip_stream_reset")

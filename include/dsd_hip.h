@@ -554,6 +554,11 @@ int dsdhip_stream_process_dev(dsdhip_stream_t *s, const float *d_pcm, int64_t pc
 /* SDM input of the last chunk into dst [nch][dst_stride] doubles (host or
  * device); returns the samples per channel or -1.  Synchronises. */
 int64_t dsdhip_stream_copy_signal(dsdhip_stream_t *s, double *dst, int64_t dst_stride);
+/* Host-side timeline of the last chunk, ms from the chunk call's start:
+ * DAS run started, then the six points of dsdhip_das_last_timeline (zero
+ * where the chunk was seeded all-parallel) shifted to the chunk's start.
+ * No reference counterpart. */
+int dsdhip_stream_last_timeline(const dsdhip_stream_t *s, double *ms7);
 
 /* ======================================================================
  * 10. DSD wire formats: DoP and raw bit bytes (include/dop.h:12-48,
