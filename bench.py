@@ -289,7 +289,10 @@ def batched_uneven(chunks, cf, plain_tracks, cpu_frames=0, cpu_threads=1):
            "config": "16 tracks x 6 ch 48 kHz f32 -> DSD512/48 (24.576 MHz, 9 stages) -> DAS K=32 L=64 B=8, "
                      "all segments of all tracks in one trellis launch per chunk"}
     if plain_tracks > 0:
-        # per-track form on the first chunk of plain_tracks tracks: FIR, then one DAS per track
+        # per-track form on the first chunk of plain_tracks tracks: FIR, then one DAS per track,
+        # with the shortest segments the reference allows (4 overlaps), which give one track's
+        # chunk the most jobs per launch
+        plain_len = 4 * 64 * L
         ch = fir.FirChain(fs_in, fs_out, fir.PCM_FIR_GAIN, nch)
         d_f = torch.empty((nch, cf * R), dtype=torch.float64, device="cuda")
         words = cf * R // 64 + 2
@@ -299,7 +302,7 @@ def batched_uneven(chunks, cf, plain_tracks, cpu_frames=0, cpu_threads=1):
             x = pcm[t][:, :cf].contiguous()
             ch.reset()
             ch.process_dev(x.data_ptr(), cf, cf, d_f.data_ptr(), cf * R)
-            D = das.Das(fi, B, K, L, nch, seg_len=tune["seg_len"], seg0_first=False, reposition=True)
+            D = das.Das(fi, B, K, L, nch, seg_len=plain_len, seg0_first=False, reposition=True)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             nout = D.run_dev(d_f.data_ptr(), cf * R, cf * R, d_o.data_ptr(), words)
@@ -307,8 +310,9 @@ def batched_uneven(chunks, cf, plain_tracks, cpu_frames=0, cpu_threads=1):
             tt += time.perf_counter() - t0
             tot += nout * nch
         res["plain_per_track"] = {"value": round(tot / tt / 1e6, 3), "unit": "Msamples/s",
+                                  "seg_len": plain_len,
                                   "sample": "first chunk of %d tracks, one DAS launch per track "
-                                            "(same segment length)" % plain_tracks}
+                                            "(segments of 4 overlaps, the most jobs per launch)" % plain_tracks}
     if cpu_frames > 0:
         x = pcm[0][:, :cpu_frames].cpu().numpy().astype(np.float32)
         v, wall, nseg = cpu_das_rate(x, fs_in, fs_out, tune["seg_len"], False, cpu_threads)

@@ -142,16 +142,54 @@ static int segs_for(int64_t F, int64_t seg_len)
     return (int)(S < 1 ? 1 : S);
 }
 
+/* Segment length for a plan: the one that minimises the modelled trellis
+ * time per useful sample over every chunk of the plan's tracks.  A chunk's
+ * launch runs its jobs in ceil(jobs / slots) rounds of its longest job
+ * (a segment's share of the chunk plus an overlap O on each side, or the
+ * whole chunk when it is one segment).  Candidates: the lengths that give a
+ * full chunk 1, 2, 3, ... segments, none shorter than 4 overlaps (the
+ * reference's minimum, src/dsp_plugin.c:1363-1367); among equal costs the
+ * longest (fewest stitches) wins. */
+static int64_t pick_seg_len(const gpu_hip_plan_t *p, int64_t O, int64_t slots)
+{
+    const int64_t Ffull = p->CF * p->R;
+    int64_t best_len = Ffull, last = -1;
+    double best_cost = 1e300;
+    for (int64_t S = 1; S <= Ffull / (4 * O) || S == 1; S++) {
+        const int64_t len = Ffull / S;
+        if (len == last || (S > 1 && len < 4 * O)) continue;
+        last = len;
+        double t_all = 0.0, useful = 0.0;
+        for (int64_t c = 0; c < p->nchunks; c++) {
+            int64_t jobs = 0, longest = 0;
+            for (int t = 0; t < p->ntracks; t++) {
+                const int64_t v = std::min(p->CF, p->tracks[(size_t)t].frames - c * p->CF);
+                if (v <= 0) continue;
+                const int64_t F = v * p->R;
+                const int Sc = segs_for(F, len);
+                jobs += (int64_t)p->nch * Sc;
+                longest = std::max(longest, Sc > 1 ? F / Sc + 2 * O : F);
+                useful += (double)F * (double)p->nch;
+            }
+            if (jobs) t_all += (double)((jobs + slots - 1) / slots) * (double)longest;
+        }
+        const double cost = useful > 0.0 ? t_all / useful : 0.0;
+        if (cost < best_cost) {
+            best_cost = cost;
+            best_len = len;
+        }
+    }
+    return best_len;
+}
+
 /* ---- autotuner ---------------------------------------------------------
  * 1. Launch shape: the persistent trellis kernel is timed on trial jobs
  *    (fresh contexts, a seeded tone-plus-noise input at the workload's
  *    level, trial_n samples each, as many jobs as resident waves) for
  *    waves-per-workgroup in {1, 2, 4} and resident waves per CU in
  *    {4, 8, 12, 16}; the fastest samples/s wins.
- * 2. Segment length (only when not given): each candidate in {16k..256k}
- *    (at least 4 overlaps) is scored with the measured throughput as the
- *    makespan of one full chunk, ceil(jobs / resident waves) rounds of
- *    seg_len + 2*O samples, per useful output sample; smallest wins.
+ * 2. Segment length (only when not given): pick_seg_len over the plan's
+ *    chunks with the resident waves as the slots.
  * K = 32 candidates are 64 children, one full wavefront per segment, so
  * candidates per wavefront is not a free parameter at this K.
  */
@@ -237,28 +275,7 @@ static int plan_tune(gpu_hip_plan_t *p)
     p->tune.waves_per_group = best_wpb;
     p->tune.waves_per_cu = best_wpc;
     p->tune.msps_measured = best_rate / 1e6;
-    if (p->tune.seg_len <= 0) {
-        const int64_t waves = (int64_t)cus * best_wpc;
-        const int64_t Ffull = p->CF * p->R;
-        double best_cost = 1e300;
-        int64_t best_len = 65536;
-        for (int64_t len = 16384; len <= 262144; len *= 2) {
-            if (len < 4 * O || len > Ffull) continue;
-            int64_t jobs = 0;
-            for (int t = 0; t < p->ntracks; t++) {
-                const int64_t v = std::min(p->CF, p->tracks[(size_t)t].frames);
-                if (v > 0) jobs += (int64_t)p->nch * segs_for(v * p->R, len);
-            }
-            const int64_t rounds = (jobs + waves - 1) / waves;
-            const double makespan = (double)rounds * (double)waves * (double)(len + 2 * O) / best_rate;
-            const double cost = makespan / ((double)jobs * (double)len);
-            if (cost < best_cost) {
-                best_cost = cost;
-                best_len = len;
-            }
-        }
-        p->tune.seg_len = (int)best_len;
-    }
+    if (p->tune.seg_len <= 0) p->tune.seg_len = (int)pick_seg_len(p, O, (int64_t)cus * best_wpc);
     rc = 0;
 done:
     hipEventDestroy(e0);
@@ -450,25 +467,7 @@ static int host_tune(gpu_hip_plan_t *p)
         if (jr == 0 && ms > 0.f) {
             const double rate = (double)nj * trial_n / (ms * 1e-3);     /* samples/s at full residency */
             p->tune.msps_measured = rate / 1e6;
-            const int64_t Ffull = p->CF * p->R;
-            double best_cost = 1e300;
-            int64_t best_len = 65536;
-            for (int64_t len = 16384; len <= 262144; len *= 2) {
-                if (len < 4 * O || len > Ffull) continue;
-                int64_t jobs = 0;
-                for (int t = 0; t < p->ntracks; t++) {
-                    const int64_t v = std::min(p->CF, p->tracks[(size_t)t].frames);
-                    if (v > 0) jobs += (int64_t)p->nch * segs_for(v * p->R, len);
-                }
-                const int64_t rounds = (jobs + slots - 1) / slots;
-                const double makespan = (double)rounds * (double)slots * (double)(len + 2 * O) / rate;
-                const double cost = makespan / ((double)jobs * (double)len);
-                if (cost < best_cost) {
-                    best_cost = cost;
-                    best_len = len;
-                }
-            }
-            p->tune.seg_len = (int)best_len;
+            p->tune.seg_len = (int)pick_seg_len(p, O, slots);
             rc = 0;
         }
     }
