@@ -380,7 +380,7 @@ int dsdhip_das_last_estimate(const dsdhip_das_t *d, double *busy_ms, int64_t *sa
  *    use_graph = 0.
  * ====================================================================== */
 typedef struct {
-    int seg_len;            /* segment length; 0: chosen by the autotuner */
+    int seg_len;            /* segment length; 0: chosen by the autotuner over the enqueued tracks */
     int waves_per_group;    /* device step's persistent launch: waves per workgroup (1, 2, 4); 0: tuned */
     int waves_per_cu;       /* device step's persistent launch: resident waves per CU; 0: tuned */
     int use_graph;          /* in: ask for the device step; out: whether it runs (0 in this build) */
