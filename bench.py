@@ -250,9 +250,10 @@ def batched_uneven(chunks, cf, plain_tracks, cpu_frames=0, cpu_threads=1):
     PCM (lengths drawn from U[5, 120] s, seeded band-limited Gaussian at
     -14 dBFS RMS) -> 9 half-band stages to DSD512/48 (24.576 MHz) -> DAS
     trellis K=32, L=64 with every segment of every channel and track in one
-    launch per chunk.  The first `chunks` chunks of `cf` frames are run (every
-    track is longer than that); the first chunk (tuning, set-up) is not
-    timed.  `plain_tracks` tracks are also run one DAS launch per track on
+    launch per chunk.  The tracks are enqueued at their full lengths (the
+    autotuner's segment length is for the whole workload); the first
+    `chunks` chunks of `cf` frames are run (every track is longer than
+    that), and the first chunk (tuning, set-up) is not timed.  `plain_tracks` tracks are also run one DAS launch per track on
     the same first chunk, for the per-track form's throughput."""
     import torch
     from foo_dsd_trellis_amd import batch, das, fir, signals, trellis
@@ -260,8 +261,15 @@ def batched_uneven(chunks, cf, plain_tracks, cpu_frames=0, cpu_threads=1):
     R = fs_out // fs_in
     lens = signals.uneven_tracks(16, nch, fs_in, seed=4)
     n = chunks * cf
-    pcm = [torch.from_numpy(signals.bandlimited_gauss(min(L_, n), fs=fs_in, nch=nch, seed=100 + t)).cuda()
-           for t, L_ in enumerate(lens)]
+    # every track is enqueued at its full length, so the autotuner sizes the
+    # segments for the whole workload; only the first n frames (the chunks
+    # that are run) carry content, the rest is silence that is never reached
+    pcm = []
+    for t, L_ in enumerate(lens):
+        x = torch.zeros((nch, L_), dtype=torch.float32, device="cuda")
+        x[:, :min(L_, n)] = torch.from_numpy(signals.bandlimited_gauss(min(L_, n), fs=fs_in, nch=nch,
+                                                                       seed=100 + t)).cuda()
+        pcm.append(x)
     fi = trellis.ntf_auto(fs_out)
     plan = batch.BatchPlan(fs_in, fs_out, fi, K=K, L=L, depth=B, nch=nch, chunk_frames=cf)
     plan.enqueue(pcm)
