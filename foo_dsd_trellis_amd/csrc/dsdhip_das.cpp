@@ -141,7 +141,10 @@ struct DasScratch {
         for (hipEvent_t e : blk) hipEventDestroy(e);
         for (hipStream_t x : bs)
             if (x) hipStreamDestroy(x);
-        if (cs) hipStreamDestroy(cs);
+        if (cs) {
+            hipStreamSynchronize(cs);     /* no staging copy may outlive h_est */
+            hipStreamDestroy(cs);
+        }
         if (ev_in) hipEventDestroy(ev_in);
         if (ev_go) hipEventDestroy(ev_go);
         if (ev_t0) hipEventDestroy(ev_t0);
@@ -690,8 +693,9 @@ static int estimate_stage(dsdhip_das_t *d, const std::vector<TrackRun> &R)
         }
     }
     if (staged == 0) return 0;
-    if (sc.ensure_host(staged) != 0 || DasScratch::grow_events(sc.blk, nblk) != 0 ||
-        hipStreamWaitEvent(sc.cs, sc.ev_in, 0) != hipSuccess)
+    /* copies of an earlier call that failed may still be in flight into h_est */
+    if (hipStreamSynchronize(sc.cs) != hipSuccess || sc.ensure_host(staged) != 0 ||
+        DasScratch::grow_events(sc.blk, nblk) != 0 || hipStreamWaitEvent(sc.cs, sc.ev_in, 0) != hipSuccess)
         return -1;
     for (size_t b = 0;; b++) {
         bool any = false;
@@ -756,7 +760,8 @@ static int seg0_first_rest(dsdhip_das_t *d, std::vector<TrackRun> &R, unsigned l
     if (total == 0) return 0;
     if (dsdhip_trellis_table_begin(tr, total) != 0) return -1;
     if (!prestaged && staged &&
-        (sc.ensure_host(staged) != 0 || DasScratch::grow_events(sc.blk, nblk) != 0))
+        (hipStreamSynchronize(sc.cs) != hipSuccess || sc.ensure_host(staged) != 0 ||
+         DasScratch::grow_events(sc.blk, nblk) != 0))
         return -1;
     if (DasScratch::grow_events(sc.bev, (size_t)total + 1) != 0) return -1;
     ent.reserve((size_t)total);
