@@ -153,19 +153,34 @@ static int segs_for(int64_t F, int64_t seg_len)
 static int64_t pick_seg_len(const gpu_hip_plan_t *p, int64_t O, int64_t slots)
 {
     const int64_t Ffull = p->CF * p->R;
+    /* chunk c holds nfull[c] whole chunks and the partial last chunks of the
+     * tracks listed in part[c] (their samples) */
+    const int64_t nc = p->nchunks;
+    std::vector<int64_t> nfull((size_t)nc + 1, 0);
+    std::vector<std::vector<int64_t>> part((size_t)nc);
+    for (int t = 0; t < p->ntracks; t++) {
+        const int64_t fr = p->tracks[(size_t)t].frames;
+        if (fr <= 0) continue;
+        const int64_t whole = std::min(fr / p->CF, nc);
+        nfull[0] += 1;
+        nfull[(size_t)whole] -= 1;
+        if (fr % p->CF && whole < nc) part[(size_t)whole].push_back((fr % p->CF) * p->R);
+    }
+    for (int64_t c = 1; c < nc; c++) nfull[(size_t)c] += nfull[(size_t)c - 1];
     int64_t best_len = Ffull, last = -1;
     double best_cost = 1e300;
     for (int64_t S = 1; S <= Ffull / (4 * O) || S == 1; S++) {
         const int64_t len = Ffull / S;
         if (len == last || (S > 1 && len < 4 * O)) continue;
         last = len;
+        const int Sf = segs_for(Ffull, len);
+        const int64_t jf = Sf > 1 ? Ffull / Sf + 2 * O : Ffull;
         double t_all = 0.0, useful = 0.0;
-        for (int64_t c = 0; c < p->nchunks; c++) {
-            int64_t jobs = 0, longest = 0;
-            for (int t = 0; t < p->ntracks; t++) {
-                const int64_t v = std::min(p->CF, p->tracks[(size_t)t].frames - c * p->CF);
-                if (v <= 0) continue;
-                const int64_t F = v * p->R;
+        for (int64_t c = 0; c < nc; c++) {
+            int64_t jobs = nfull[(size_t)c] * (int64_t)p->nch * Sf;
+            int64_t longest = nfull[(size_t)c] ? jf : 0;
+            useful += (double)nfull[(size_t)c] * (double)Ffull * (double)p->nch;
+            for (const int64_t F : part[(size_t)c]) {
                 const int Sc = segs_for(F, len);
                 jobs += (int64_t)p->nch * Sc;
                 longest = std::max(longest, Sc > 1 ? F / Sc + 2 * O : F);
