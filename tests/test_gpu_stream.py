@@ -55,6 +55,12 @@ def test_stream_matches_oracle(gpu, oracle, with_ir):
         want = das.chunk(sig, SEG, int(S.seg0_first), 1)
         assert n == want.shape[1], (f0, n, want.shape)
         assert np.array_equal(S.bits_numpy(n), want), "chunk at frame %d" % f0
+        # host timeline of the chunk: DAS start, then the points the run
+        # passed (zero otherwise), in order, ending with the DAS return
+        tl = S.last_timeline()
+        assert len(tl) == 7 and 0.0 < tl[0] <= tl[6], tl
+        passed = [t for t in tl if t > 0.0]
+        assert passed == sorted(passed), tl
         total += n
     assert total > (frames * R) * 0.9
 
