@@ -136,22 +136,38 @@ __device__ __forceinline__ unsigned shl_add_ltu(unsigned acc, unsigned k, unsign
 /* Strict-order mask on the metrics' high words: for non-negative doubles
  * the order of the bit patterns is the order of the values, so where all
  * high words differ this is the exact strict order.  16 keys per batch
- * (four 128-bit loads). */
-template <int NJ>
+ * (four 128-bit loads).  With ALL_LOADS every batch is loaded before the
+ * first compare (one LDS round trip, 4 registers a key: for a wave that
+ * has its SIMD to itself); otherwise batch by batch (registers for 16
+ * keys; the other waves of the SIMD cover the round trips). */
+template <int NJ, bool ALL_LOADS = false>
 __device__ __forceinline__ u64 lt_sweep_hi(const uint32_t *key, unsigned c)
 {
     const uint4 *k4 = reinterpret_cast<const uint4 *>(key);
     unsigned w[2] = {0u, 0u};
+    if constexpr (ALL_LOADS) {
+        uint4 kv[NJ / 4];
 #pragma unroll
-    for (int b = NJ / 16 - 1; b >= 0; b--) {
-        uint4 kv[4];
+        for (int i = NJ / 4 - 1; i >= 0; i--) kv[i] = k4[i];
 #pragma unroll
-        for (int i = 3; i >= 0; i--) kv[i] = k4[b * 4 + i];
-        unsigned acc = w[b >> 1];
+        for (int b = NJ / 16 - 1; b >= 0; b--) {
+            unsigned acc = w[b >> 1];
 #pragma unroll
-        for (int i = 3; i >= 0; i--) acc = shl_add_ltu4(acc, kv[i], c);
-        w[b >> 1] = acc;
-        __builtin_amdgcn_sched_barrier(0);
+            for (int i = 3; i >= 0; i--) acc = shl_add_ltu4(acc, kv[b * 4 + i], c);
+            w[b >> 1] = acc;
+        }
+    } else {
+#pragma unroll
+        for (int b = NJ / 16 - 1; b >= 0; b--) {
+            uint4 kv[4];
+#pragma unroll
+            for (int i = 3; i >= 0; i--) kv[i] = k4[b * 4 + i];
+            unsigned acc = w[b >> 1];
+#pragma unroll
+            for (int i = 3; i >= 0; i--) acc = shl_add_ltu4(acc, kv[i], c);
+            w[b >> 1] = acc;
+            __builtin_amdgcn_sched_barrier(0);
+        }
     }
     return ((u64)w[1] << 32) | w[0];
 }
@@ -388,7 +404,7 @@ __device__ int wide_prune(TrLds<P, HW> &s, int nb, int m, int K, const tr_ctx *s
 /* One job on one wave; s is the wave's own LDS region.  Nothing here
  * synchronises beyond the wave, so several waves of a workgroup may run
  * different jobs (and different numbers of them). */
-template <int P, int HW, bool F8 = false, bool UJ = false, int LIM = 2>
+template <int P, int HW, bool F8 = false, bool UJ = false, int LIM = 2, bool LONE = false>
 __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr_ctx *ctxs,
                                             TrLds<P, HW> &s, const int lane)
 {
@@ -623,8 +639,8 @@ __device__ __forceinline__ void tr_job_body(const tr_params &prm, tr_job *jb, tr
         {
             const bool neg = (ballot((int)chi < 0) & ALL) != 0;
             if (!neg) {
-                LT = (m <= 16) ? lt_sweep_hi<16>(s.khi, chi)
-                   : (m <= 32) ? lt_sweep_hi<32>(s.khi, chi) : lt_sweep_hi<64>(s.khi, chi);
+                LT = (m <= 16) ? lt_sweep_hi<16, LONE>(s.khi, chi)
+                   : (m <= 32) ? lt_sweep_hi<32, LONE>(s.khi, chi) : lt_sweep_hi<64, LONE>(s.khi, chi);
                 if (groups_ok) {
                     /* candidates and groups from these masks, the tie test in
                      * the same exchange; with a tie all of it is redone below */
@@ -901,6 +917,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TrWaves<HW>:
     else tr_job_body<P, HW, false, false, 0>(prm, jb, ctxs, s, threadIdx.x);
 }
 
+/* The same for launches of a few jobs (segment 0 of a seg0-first step, a
+ * plain stream), each of which has a SIMD to itself: registers are not
+ * rationed for occupancy, and the sweep loads all its keys at once. */
+template <int P, int HW>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 8))) void tr_run_jobs_lone(tr_params prm, tr_job *jobs, tr_ctx *ctxs)
+{
+    __shared__ TrLds<P, HW> s;
+    tr_job *jb = &jobs[blockIdx.x];
+    if (jb->skip) return;
+    if (prm.state_limit > 0.0) tr_job_body<P, HW, false, false, 1, true>(prm, jb, ctxs, s, threadIdx.x);
+    else tr_job_body<P, HW, false, false, 0, true>(prm, jb, ctxs, s, threadIdx.x);
+}
+
 /* The same with the order-8 FMA filter of the reference's AVX2 build. */
 template <int HW>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TrWaves<HW>::value, 8))) void tr_run_jobs_f8(tr_params prm, tr_job *jobs, tr_ctx *ctxs)
@@ -1041,6 +1070,16 @@ TF_INST(6)
 TF_INST(7)
 TF_INST(8)
 
+/* the few-job form where the occupancy target applies (TrWaves above 1) */
+#define TL_INST(P) \
+    template __global__ void tr_run_jobs_lone<P, 1>(tr_params, tr_job *, tr_ctx *); \
+    template __global__ void tr_run_jobs_lone<P, 2>(tr_params, tr_job *, tr_ctx *);
+TL_INST(4)
+TL_INST(5)
+TL_INST(6)
+TL_INST(7)
+TL_INST(8)
+
 #define TR_INST(P, HW) \
     template __global__ void tr_run_jobs<P, HW>(tr_params, tr_job *, tr_ctx *); \
     template __global__ void tr_run_queue<P, HW, 1>(tr_params, tr_job *, int, int *, tr_ctx *); \
@@ -1066,6 +1105,16 @@ extern "C" int tr_lane_launch(const tr_params *prm, tr_job *d_jobs, int njobs, t
 /* Host-side launcher table (P in 4..8, HW in {1,2,4,8,16,32}).  At K <= 2
  * the jobs run one per lane (tr_lane_kernel.hip) unless prm asks for the
  * one-wave-per-job kernel. */
+/* compute units of the current device */
+static int tr_cu_count()
+{
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+        return 0;
+    return n;
+}
+
 extern "C" int tr_launch_jobs(const tr_params *prm, tr_job *d_jobs, int njobs,
                               tr_ctx *d_ctxs, hipStream_t stream)
 {
@@ -1083,6 +1132,16 @@ extern "C" int tr_launch_jobs(const tr_params *prm, tr_job *d_jobs, int njobs,
             hipLaunchKernelGGL((tr_run_jobs_f8<HH>), grid, block, 0, stream, *prm, d_jobs, d_ctxs); \
             return hipGetLastError() == hipSuccess ? 0 : -1; }
         TF_CASE(1) TF_CASE(2) TF_CASE(4) TF_CASE(8) TF_CASE(16) TF_CASE(32)
+        return -1;
+    }
+    /* a launch of at most one job per CU gives each job a SIMD of its own */
+    if (HW <= 2 && njobs <= tr_cu_count()) {
+#define TL_CASE(PP, HH) \
+        if (P == PP && HW == HH) { \
+            hipLaunchKernelGGL((tr_run_jobs_lone<PP, HH>), grid, block, 0, stream, *prm, d_jobs, d_ctxs); \
+            return hipGetLastError() == hipSuccess ? 0 : -1; }
+        TL_CASE(4, 1) TL_CASE(4, 2) TL_CASE(5, 1) TL_CASE(5, 2) TL_CASE(6, 1) TL_CASE(6, 2)
+        TL_CASE(7, 1) TL_CASE(7, 2) TL_CASE(8, 1) TL_CASE(8, 2)
         return -1;
     }
 #define TR_CASE(PP, HH) \
