@@ -1423,7 +1423,8 @@ int dsdhip_das_last_timeline(const dsdhip_das_t *d, double *ms6)
 }
 
 /* Resident waves per CU of the host-fed launch (default 6, one or two per
- * SIMD; 0: as many as the kernel's registers allow).  The segments arrive
+ * SIMD; 0: as many as the kernel's registers allow, 8 up to 128-sample
+ * histories).  The segments arrive
  * at the estimator's pace, which a few waves per SIMD keep up with; fewer
  * resident waves leave the last segments fewer neighbours on their SIMD, so
  * the step ends sooner after the last estimate, until published segments

@@ -1013,8 +1013,16 @@ static_assert(offsetof(tr_job, slow_steps) == offsetof(tr_job, kept) + 8 &&
  * workgroups are dispatched in index order, so the waiting workgroups are
  * the next ones to run; a wave leaves early on the host's abort flag or
  * after a wait longer than `limit`, so the grid always drains. */
+/* The host-fed launch holds at most a few waves per CU (its LDS request is
+ * padded to 6 per CU by default): registers for two waves per SIMD, which
+ * the sweep spends on loading all its keys at once. */
+template <int HW>
+struct TrFeedWaves {
+    static constexpr int value = HW <= 2 ? 2 : 1;
+};
+
 template <int P, int HW>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TrWaves<HW>::value, 8))) void tr_run_feed(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TrFeedWaves<HW>::value, 8))) void tr_run_feed(
     tr_params prm, const tr_job *hjobs, tr_job *jobs, const tr_feed_host *hc, tr_feed_dev *dc, tr_ctx *ctxs,
     uint64_t poll, uint64_t limit, uint64_t max_iter)
 {
@@ -1038,8 +1046,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TrWaves<HW>:
             reinterpret_cast<const uint64_t *>(&hjobs[j]) + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     wsync();
     if (!uni(jd.skip)) {
-        if (prm.state_limit > 0.0) tr_job_body<P, HW, false, true, 1>(prm, &jd, ctxs, s, lane);
-        else tr_job_body<P, HW, false, true, 0>(prm, &jd, ctxs, s, lane);
+        if (prm.state_limit > 0.0) tr_job_body<P, HW, false, true, 1, true>(prm, &jd, ctxs, s, lane);
+        else tr_job_body<P, HW, false, true, 0, true>(prm, &jd, ctxs, s, lane);
     }
     wsync();
     if (lane < NW) reinterpret_cast<uint64_t *>(&jobs[j])[lane] = reinterpret_cast<const uint64_t *>(&jd)[lane];

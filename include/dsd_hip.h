@@ -347,7 +347,8 @@ int dsdhip_das_batch_launch(dsdhip_das_t *d, int on);
  * No reference counterpart. */
 int dsdhip_das_feed_limit(dsdhip_das_t *d, double seconds);
 /* Resident waves per CU of the host-fed launch (default 6; 0: as many as
- * the kernel's registers allow).  Fewer waves per SIMD run the last
+ * the kernel's registers allow, 8 up to 128-sample histories).  Fewer
+ * waves per SIMD run the last
  * published segments faster; too few leave published segments waiting for
  * a slot.  No reference counterpart. */
 int dsdhip_das_feed_waves(dsdhip_das_t *d, int waves_per_cu);
